@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the staggered hot path on one MI355X.
+
+Headline (BASELINE metric): HISQ Dslash GB/s and fraction of HBM peak on a 32^4
+lattice, plus FP64 vs mixed-precision CG time to a 1e-10 true residual.
+
+A "step" is one single-parity even-odd Dslash over the 32^4 lattice's 524288
+even sites (FP64, fat 1-hop + Naik 3-hop, Naik links stored as 12 reals),
+inputs resident in HBM.  `value` is algorithmic bytes / average Dslash time,
+with 2017 bytes per site: 8 fat links x 18 x 8 B + 8 Naik links x 12 x 8 B +
+1 B of Naik row-3 signs + 48 B input + 48 B output.
+
+The CPU baseline is the oracle's OpenMP Dslash (same site loop as the
+reference's ks_long_dslash.cpp:564-668) on a bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "milc-qphix_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "HISQ Dslash GB/s (% HBM peak); mixed vs FP64 CG time to 1e-10 residual, 32^4"
+HBM_PEAK_GBS = 8000.0
+DIMS = (32, 32, 32, 32)
+
+
+def site_bytes(real_bytes, n_long, spinor_bytes):
+    """Algorithmic bytes per output site of one single-parity Dslash."""
+    sign = 1 if n_long == 12 else 0
+    return real_bytes * (8 * 18 + 8 * n_long) + sign + 2 * spinor_bytes
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_fields(lat, F):
+    fat, lng = F.make_links(lat, "weak", 1002, eps=0.3)
+    raw = F.raw_link_set(lat, fat, lng)
+    del fat, lng
+    src = F.to_raw(F.gaussian_source(lat, 2002))
+    return raw, src
+
+
+def cpu_baseline(dims, raw, src, budget_s):
+    """Oracle Dslash (FP64) timed on the host: bounded number of applications."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    orc = O.Oracle(dims, *raw)
+    vh = int(np.prod(dims)) // 2
+    half = np.ascontiguousarray(src[6 * vh:])
+    t = time.perf_counter()
+    orc.dslash(half, 0)  # warm-up
+    first = time.perf_counter() - t
+    reps = max(2, min(50, int(budget_s / max(first, 1e-3))))
+    t = time.perf_counter()
+    for _ in range(reps):
+        orc.dslash(half, 0)
+    dt = (time.perf_counter() - t) / reps
+    threads = O.lib().oracle_num_threads()
+    orc.close()
+    return dt, reps, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one lattice on one GPU; --gpus {args.gpus} is not "
+              "supported (multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    from pyqphix import api as A
+    from pyqphix import fields as F
+
+    lat = F.Lattice(DIMS)
+    log("[bench] generating 32^4 weak-field links")
+    t = time.perf_counter()
+    raw, src = make_fields(lat, F)
+    log(f"[bench] fields ready in {time.perf_counter() - t:.1f}s")
+
+    s = A.Session(DIMS)
+    lib = s.lib
+    ncu = __import__("ctypes").c_int()
+    lib.QPHIX_MI355X_device_info(__import__("ctypes").byref(ncu), None)
+
+    # achieved copy bandwidth (read + write), for reference
+    copy_bytes = 2 << 30
+    copy_ms = lib.QPHIX_MI355X_copy_time(copy_bytes, 20)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+
+    L64 = A.Links(s, *raw, prec="D")
+    vin = A.Vector(s, src, A.EVENODD, "D")
+    vout = A.Vector(s, None, A.EVENODD, "D")
+    rb = site_bytes(8, L64.recon, 48)
+    log(f"[bench] FP64 links recon={L64.recon}, {rb} B/site")
+
+    A.dslash_time(L64, vout, vin, A.EVEN, args.warmup)
+    lib.QPHIX_MI355X_sync()
+    t0 = time.perf_counter()
+    ms_ev = A.dslash_time(L64, vout, vin, A.EVEN, args.steps)
+    lib.QPHIX_MI355X_sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    gbs = rb * lat.vh / (ms_ev * 1e-3) / 1e9
+    gflops = 1146.0 * lat.vh / (ms_ev * 1e-3) / 1e9
+    log(f"[bench] FP64 Dslash {ms_ev * 1e3:.1f} us  {gbs:.0f} GB/s  {gflops:.0f} GFLOP/s")
+
+    extra = {"copy_kernel_GBs": round(copy_gbs, 1), "n_cu": ncu.value}
+    extra["dslash_f64"] = {"us": round(ms_ev * 1e3, 2), "GBs": round(gbs, 1),
+                           "GFLOPs": round(gflops, 1), "bytes_per_site": rb, "recon": L64.recon}
+
+    # FP32 Dslash (the mixed-precision solver's operator)
+    raw32 = [r.astype(np.float32) for r in raw]
+    L32 = A.Links(s, *raw32, prec="F")
+    vin32 = A.Vector(s, src.astype(np.float32), A.EVENODD, "F")
+    vout32 = A.Vector(s, None, A.EVENODD, "F")
+    A.dslash_time(L32, vout32, vin32, A.EVEN, args.warmup)
+    ms32 = A.dslash_time(L32, vout32, vin32, A.EVEN, args.steps)
+    rb32 = site_bytes(4, L32.recon, 24)
+    extra["dslash_f32"] = {"us": round(ms32 * 1e3, 2),
+                           "GBs": round(rb32 * lat.vh / (ms32 * 1e-3) / 1e9, 1),
+                           "bytes_per_site": rb32, "recon": L32.recon}
+    log(f"[bench] FP32 Dslash {ms32 * 1e3:.1f} us  {extra['dslash_f32']['GBs']} GB/s")
+    del raw32
+
+    if not args.no_cg:
+        # FP64 CG to a 1e-10 true relative residual (resid is squared: 1e-20), m = 0.02
+        mass = 0.01  # API mass m/2 for M = m^2 - D_eo D_oe with m = 0.02
+        b = src.copy()
+        b[6 * lat.vh:] = 0
+        vb = A.Vector(s, b, A.EVEN, "D")
+        vx = A.Vector(s, None, A.EVEN, "D")
+        t = time.perf_counter()
+        ret, info, res = A.invert(L64, vx, vb, mass, A.EVEN, 1e-20, niter=5000, nrestart=10)
+        cg_s = time.perf_counter() - t
+        extra["cg_f64"] = {"seconds": round(cg_s, 4), "iters": res.final_iter,
+                           "final_rsq": res.final_rsq, "restarts": res.final_restart}
+        log(f"[bench] FP64 CG {cg_s:.3f}s {res.final_iter} iters rsq {res.final_rsq:.2e}")
+        vb.free()
+        vx.free()
+
+    cpu = None
+    if not args.no_cpu:
+        dt, reps, threads = cpu_baseline(DIMS, raw, src, args.cpu_seconds)
+        cpu = {"value": round(rb * lat.vh / dt / 1e9, 2), "unit": "GB/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{reps} FP64 single-parity Dslash applications on the 32^4 weak-field "
+                         f"lattice, oracle OpenMP site loop, {dt * 1e3:.1f} ms each"}
+        log(f"[bench] CPU oracle Dslash {dt * 1e3:.1f} ms ({threads} threads)")
+
+    out = {
+        "metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "32^4 even-odd HISQ/asqtad Dslash (fat 1-hop + Naik 3-hop), "
+                               "weak-field links eps=0.3, FP64, Naik links as 12 reals",
+                   "lattice": list(DIMS), "sites_per_step": lat.vh},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    for o in (L64, vin, vout, L32, vin32, vout32):
+        o.free()
+    s.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,69 @@
+/* qphix_mi355x.h -- entry points of libqphixmilc_hip.so that have no
+ * counterpart in the reference's MILC interface.
+ *
+ * They are additions for callers that want the MI355X-specific solvers
+ * (mixed precision with reliable updates, batched right-hand sides), the
+ * fermion force, or control over synchronisation and diagnostics.  Plain C,
+ * pointers and sizes only.
+ */
+#ifndef _QPHIX_MI355X_H
+#define _QPHIX_MI355X_H
+
+#include <qphix_int.h>
+#include <qphix_f3.h>
+#include <qphix_d3.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- run-time control and diagnostics ------------------------------------ */
+
+/* on != 0 (default): a contract violation or HIP failure prints and exits,
+ * as the reference's MYASSERT does.  on == 0: the failure is recorded and the
+ * call returns (QPHIX_FAIL where the signature allows). */
+void QPHIX_MI355X_set_abort_on_error(int on);
+/* Number of failures recorded since the last clear; copies the last message. */
+int QPHIX_MI355X_last_error(char *buf, int len);
+void QPHIX_MI355X_clear_error(void);
+
+/* on != 0 (default): every call has finished on the GPU when it returns.
+ * on == 0: Dslash calls return once queued on the library's stream; solvers
+ * and extract_* still synchronise before returning host data. */
+void QPHIX_MI355X_set_sync(int on);
+void QPHIX_MI355X_sync(void);
+/* The hipStream_t every kernel of the library runs on. */
+void *QPHIX_MI355X_stream(void);
+int QPHIX_MI355X_device_info(int *n_cu, double *hbm_gib);
+
+/* Long-link storage: 0 = automatic (12 reals when every long link equals
+ * +-conj(row0 x row1) in its third row, else 18), 18 = always 18.
+ * Applies to links created afterwards. */
+void QPHIX_MI355X_set_long_recon(int mode);
+/* Storage chosen for L: 12 or 18. */
+int QPHIX_D3_asqtad_links_recon(QPHIX_D3_FermionLinksAsqtad *L);
+int QPHIX_F3_asqtad_links_recon(QPHIX_F3_FermionLinksAsqtad *L);
+
+/* Back-link convention probe (needs both parities).  Returns the largest
+ * |back_mu(x) - s * U_mu(x - hop)^dagger| over all back links, for
+ * s = -1 in out[0] and s = +1 in out[1], fat links in out[0..1] and long
+ * links in out[2..3]. */
+void QPHIX_D3_asqtad_links_backcheck(QPHIX_D3_FermionLinksAsqtad *L, double out[4]);
+
+/* ---- timing ----------------------------------------------------------------
+ * Runs `reps` single-parity Dslash applications back to back on the library's
+ * stream and returns the average kernel time in milliseconds measured with
+ * HIP events on that stream. */
+double QPHIX_D3_asqtad_dslash_time(QPHIX_D3_FermionLinksAsqtad *L, QPHIX_D3_ColorVector *out,
+                                   QPHIX_D3_ColorVector *in, QPHIX_evenodd_t parity, int reps);
+double QPHIX_F3_asqtad_dslash_time(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVector *out,
+                                   QPHIX_F3_ColorVector *in, QPHIX_evenodd_t parity, int reps);
+/* Plain device-to-device copy of `bytes` bytes, `reps` times; average ms.
+ * Used to calibrate achieved HBM bandwidth and profiler byte counters. */
+double QPHIX_MI355X_copy_time(size_t bytes, int reps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* _QPHIX_MI355X_H */

@@ -1,0 +1,226 @@
+// blas.hip -- fused vector updates and reductions of the CG and multi-shift
+// solvers.  One work-item per site (6 reals); FP64 accumulation; the grid
+// reduction's last block runs the solver's scalar update on the device.
+//
+// Reference loops: calc_source_norm / calc_rsq2 / axpy (ks_d_congrad_fn.cpp:62-188),
+// calc_rsq2 / update_dest / update_pm / copyKS (ks_multicg_offset.cpp:69-184).
+#include "blas.h"
+
+namespace qphix {
+
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK) k_norm2(const T *v, int vh, ReduceCtl rc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[1] = {0.0};
+  if (i < vh) {
+    cplx<typename Spin<T>::C> a[3];
+    Spin<T>::load(v, i, a);
+    for (int c = 0; c < 3; ++c) s[0] += (double)a[c].re * a[c].re + (double)a[c].im * a[c].im;
+  }
+  grid_reduce<1>(s, rc);
+}
+
+// x += a p ; r += a ttt ; reduce |r|^2 and the per-site relative residue.
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_cg_update(T *x, const T *p, T *r, const T *ttt, int vh, ReduceCtl rc, const int *gate) {
+  if (*gate == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[2] = {0.0, 0.0};
+  if (i < vh) {
+    using C = typename Spin<T>::C;
+    const C a = (C)rc.state->a;
+    cplx<C> xv[3], pv[3], rv[3], tv[3];
+    Spin<T>::load(x, i, xv);
+    Spin<T>::load(p, i, pv);
+    Spin<T>::load(r, i, rv);
+    Spin<T>::load(ttt, i, tv);
+    double num = 0, den = 0;
+    for (int c = 0; c < 3; ++c) {
+      xv[c].re = xv[c].re + a * pv[c].re;
+      xv[c].im = xv[c].im + a * pv[c].im;
+      rv[c].re = rv[c].re + a * tv[c].re;
+      rv[c].im = rv[c].im + a * tv[c].im;
+      num += (double)rv[c].re * rv[c].re + (double)rv[c].im * rv[c].im;
+      den += (double)xv[c].re * xv[c].re + (double)xv[c].im * xv[c].im;
+    }
+    Spin<T>::store(x, i, xv);
+    Spin<T>::store(r, i, rv);
+    s[0] = num;
+    s[1] = num == 0 ? 0.0 : (den == 0 ? 1.0 : num / den);
+  }
+  grid_reduce<2>(s, rc);
+}
+
+// p = r + b p
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_cg_axpy(const T *r, T *p, int vh, const SolverState *st, const int *gate) {
+  if (*gate == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  using C = typename Spin<T>::C;
+  const C b = (C)st->b;
+  cplx<C> rv[3], pv[3];
+  Spin<T>::load(r, i, rv);
+  Spin<T>::load(p, i, pv);
+  for (int c = 0; c < 3; ++c) {
+    pv[c].re = rv[c].re + b * pv[c].re;
+    pv[c].im = rv[c].im + b * pv[c].im;
+  }
+  Spin<T>::store(p, i, pv);
+}
+
+// multi-shift: r += beta_low ttt ; reduce |r|^2
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_ms_update_r(T *r, const T *ttt, int vh, ReduceCtl rc, const int *gate) {
+  if (*gate == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[1] = {0.0};
+  if (i < vh) {
+    using C = typename Spin<T>::C;
+    const C bl = (C)rc.state->beta_i[rc.state->j_low];
+    cplx<C> rv[3], tv[3];
+    Spin<T>::load(r, i, rv);
+    Spin<T>::load(ttt, i, tv);
+    for (int c = 0; c < 3; ++c) {
+      rv[c].re = rv[c].re + bl * tv[c].re;
+      rv[c].im = rv[c].im + bl * tv[c].im;
+      s[0] += (double)rv[c].re * rv[c].re + (double)rv[c].im * rv[c].im;
+    }
+    Spin<T>::store(r, i, rv);
+  }
+  grid_reduce<1>(s, rc);
+}
+
+// multi-shift update_pm: x_j += beta_j p_j ; p_j = zeta_j r + alpha_j p_j,
+// every shift j < num_now in one launch (blockIdx.y = j).
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_ms_update_pm(T *const *x, T *const *p, const T *r, int vh, const SolverState *st,
+                   const int *gate) {
+  if (*gate == 0) return;
+  const int j = blockIdx.y;
+  if (j >= st->num_now) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  using C = typename Spin<T>::C;
+  const C bj = (C)st->beta_i[j], aj = (C)st->alpha[j];
+  const double zj = st->zeta_ip1[j];
+  cplx<C> xv[3], pv[3], rv[3];
+  Spin<T>::load(x[j], i, xv);
+  Spin<T>::load(p[j], i, pv);
+  Spin<T>::load(r, i, rv);
+  for (int c = 0; c < 3; ++c) {
+    xv[c].re = xv[c].re + bj * pv[c].re;
+    xv[c].im = xv[c].im + bj * pv[c].im;
+    const C tr = (C)((double)rv[c].re * zj), ti = (C)((double)rv[c].im * zj);
+    pv[c].re = tr + aj * pv[c].re;
+    pv[c].im = ti + aj * pv[c].im;
+  }
+  Spin<T>::store(x[j], i, xv);
+  Spin<T>::store(p[j], i, pv);
+}
+
+// multi-shift update_dest: x_j += beta_j p_j, j < num_now
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_ms_update_dest(T *const *x, T *const *p, int vh, const SolverState *st) {
+  const int j = blockIdx.y;
+  if (j >= st->num_now) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  using C = typename Spin<T>::C;
+  const C bj = (C)st->beta_i[j];
+  cplx<C> xv[3], pv[3];
+  Spin<T>::load(x[j], i, xv);
+  Spin<T>::load(p[j], i, pv);
+  for (int c = 0; c < 3; ++c) {
+    xv[c].re = xv[c].re + bj * pv[c].re;
+    xv[c].im = xv[c].im + bj * pv[c].im;
+  }
+  Spin<T>::store(x[j], i, xv);
+}
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
+// --------------------------------------------------------------- launchers
+static inline int nb(int vh) { return (vh + BLAS_BLOCK - 1) / BLAS_BLOCK; }
+
+ReduceCtl reduce_ctl(SolverState *st, int op) {
+  ReduceCtl rc;
+  rc.partials = ctx().partials;
+  rc.counter = ctx().counter;
+  rc.state = st;
+  rc.op = op;
+  rc.out = st ? nullptr : ctx().partials + 4 * (size_t)(ctx().max_blocks - 1);
+  return rc;
+}
+
+double *reduce_result_ptr() { return ctx().partials + 4 * (size_t)(ctx().max_blocks - 1); }
+
+template <typename T> void blas_norm2(const T *v, SolverState *st, int op) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_norm2<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, v, vh,
+                     reduce_ctl(st, op));
+  HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void blas_cg_update(T *x, const T *p, T *r, const T *ttt, SolverState *st, const int *gate) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_cg_update<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, x, p, r,
+                     ttt, vh, reduce_ctl(st, SOP_CG_RSQ2), gate);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <typename T> void blas_cg_axpy(const T *r, T *p, const SolverState *st, const int *gate) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_cg_axpy<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, r, p, vh, st,
+                     gate);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <typename T> void blas_ms_update_r(T *r, const T *ttt, SolverState *st, const int *gate) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_ms_update_r<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, r, ttt, vh,
+                     reduce_ctl(st, SOP_MS_RSQ2), gate);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void blas_ms_update_pm(T *const *x, T *const *p, const T *r, int nshift, const SolverState *st,
+                       const int *gate) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_ms_update_pm<T>, dim3(nb(vh), nshift), dim3(BLAS_BLOCK), 0, ctx().stream,
+                     x, p, r, vh, st, gate);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void blas_ms_update_dest(T *const *x, T *const *p, int nshift, const SolverState *st) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_ms_update_dest<T>, dim3(nb(vh), nshift), dim3(BLAS_BLOCK), 0, ctx().stream,
+                     x, p, vh, st);
+  HIP_CHECK(hipGetLastError());
+}
+
+void set_int(int *p, int v) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, ctx().stream, p, v);
+  HIP_CHECK(hipGetLastError());
+}
+
+#define INST(T)                                                                                \
+  template void blas_norm2<T>(const T *, SolverState *, int);                                  \
+  template void blas_cg_update<T>(T *, const T *, T *, const T *, SolverState *, const int *); \
+  template void blas_cg_axpy<T>(const T *, T *, const SolverState *, const int *);             \
+  template void blas_ms_update_r<T>(T *, const T *, SolverState *, const int *);               \
+  template void blas_ms_update_pm<T>(T *const *, T *const *, const T *, int,                   \
+                                     const SolverState *, const int *);                        \
+  template void blas_ms_update_dest<T>(T *const *, T *const *, int, const SolverState *);
+INST(double)
+INST(float)
+#undef INST
+
+} // namespace qphix

@@ -1,0 +1,166 @@
+// common.h -- internal declarations shared by the HIP sources of
+// libqphixmilc_hip.so.  Not installed; callers see include/qphix*.h only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+
+extern "C" {
+#include <qphix_int.h>
+#include <qphix_f3.h>
+#include <qphix_d3.h>
+}
+
+namespace qphix {
+
+// ---------------------------------------------------------------------------
+// Error handling.  Contract violations and HIP failures either abort the
+// process (the reference's MYASSERT behaviour, ks_boundary.h:35) or, when the
+// caller switched abort-on-error off, are recorded for QPHIX_MI355X_last_error.
+// ---------------------------------------------------------------------------
+void fail(const char *file, int line, const char *msg);
+bool failed();
+
+#define QPHIX_CHECK(cond, msg)                                   \
+  do {                                                           \
+    if (!(cond)) ::qphix::fail(__FILE__, __LINE__, msg);         \
+  } while (0)
+
+// Fails and returns from the calling function (optional return value).
+#define QPHIX_REQUIRE(cond, msg, ...)                            \
+  do {                                                           \
+    if (!(cond)) {                                               \
+      ::qphix::fail(__FILE__, __LINE__, msg);                    \
+      return __VA_ARGS__;                                        \
+    }                                                            \
+  } while (0)
+
+#define HIP_CHECK(expr)                                          \
+  do {                                                           \
+    hipError_t e_ = (expr);                                      \
+    if (e_ != hipSuccess) ::qphix::fail(__FILE__, __LINE__, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Geometry of the single global lattice (one lattice per process, as in the
+// reference, ks_globals.h:14-43).  Site index within a parity is MILC's raw
+// index i = xh + nxh*(y + ny*(z + nz*t)); full x = 2*xh + ((y+z+t+cb)&1).
+// ---------------------------------------------------------------------------
+struct Geometry {
+  int nx, ny, nz, nt, nxh;
+  int vh;   // sites per parity
+  int nxy;  // nxh*ny   (stride of z)
+  int nxyz; // nxh*ny*nz (stride of t)
+};
+
+struct Context {
+  bool initialised = false;
+  Geometry g{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool sync_calls = true;    // every API call synchronises before returning
+  bool abort_on_error = true;
+  int n_cu = 0;
+  // reduction scratch (device): partial sums, block counter
+  double *partials = nullptr;
+  unsigned *counter = nullptr;
+  int max_blocks = 0;
+  // pinned host mirror for small scalar reads
+  void *pinned = nullptr;
+  // workspace cache for solver temporaries
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+};
+
+Context &ctx();
+bool require_init();
+void sync_if_needed();
+void *ws_get(size_t bytes);  // grows; valid until the next ws_get with more bytes
+void dev_free(void *p);
+void *dev_alloc(size_t bytes);
+
+// ---------------------------------------------------------------------------
+// Device field storage.
+//
+// Colour vector: per parity, vh sites x 6 reals, [colour][re,im] -- the raw
+// MILC order, so create/extract are plain copies.
+//
+// Asqtad links of output parity cb: for each direction mu the forward link
+// (slot 2mu+1) and the back link (slot 2mu) form a "pair" of NC complex numbers
+// (NC = 18, or 12 when only rows 0-1 of each long link are stored).  A pair is
+// cut into 16-byte chunks (1 complex double or 2 complex float) and stored
+// chunk-major over sites: elem(mu, chunk, i) at ((mu*NCH + chunk)*vh + i).
+// Consecutive lanes load consecutive 16 B -> every link load is a full
+// coalesced 1 KiB wavefront transaction.
+// ---------------------------------------------------------------------------
+template <typename T> struct vec16;
+template <> struct vec16<double> { using type = double2; static constexpr int ncplx = 1; };
+template <> struct vec16<float> { using type = float4; static constexpr int ncplx = 2; };
+
+template <typename T> struct LinksT {
+  T *fat[2] = {nullptr, nullptr};
+  T *lng[2] = {nullptr, nullptr};
+  uint8_t *lsign[2] = {nullptr, nullptr};  // 12-recon row-2 sign, bit (2mu+fwd)
+  int recon = 18;                          // 18 or 12 (long links)
+  int parity = 0;                          // QPHIX_EVEN/ODD/EVENODD supplied
+  LinksT<float> *sloppy = nullptr;         // lazily built FP32 copy (D3 only)
+};
+
+template <typename T> struct VectorT {
+  T *half[2] = {nullptr, nullptr};  // [0] even sites, [1] odd sites
+  int parity = 0;
+};
+
+// Internal gauge field (8 slots per even site, reference qphix_d3.cpp:177-343)
+// and momentum (8 reals per link, C13 mapping) kept in raw per-parity order on
+// the device: gauge[cb][i][mu][3][3][2], mom[cb][i][mu][8].
+template <typename T> struct GaugeT {
+  T *u[2] = {nullptr, nullptr};
+  int parity = 0;
+};
+template <typename T> struct ForceT {
+  T *h[2] = {nullptr, nullptr};
+  int parity = 0;
+};
+
+// Half-precision colour vector used by the FP16-stored-spinor solver: 6 halves
+// of a per-site-normalised vector plus the float norm, 16 bytes per site.
+struct alignas(16) HalfSite {
+  __half v[6];
+  float norm;
+};
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (defined in the .hip files).
+// ---------------------------------------------------------------------------
+enum DslashEpilogue {
+  EPI_PLAIN = 0,  // out = D in
+  EPI_PKP = 1,    // out = D in + s*aux ; reduce aux.out
+  EPI_RESTART = 2 // r = b + (D in + s*aux) ; p = r ; reduce |r|^2 (+ rel)
+};
+
+struct DslashExtra {
+  double s = 0;                // scalar of the epilogue (-msq_x4 or -sigma)
+  const void *aux = nullptr;   // p (PKP) or x (RESTART)
+  const void *b = nullptr;     // RESTART: source
+  void *r = nullptr;           // RESTART: residual out
+  void *p = nullptr;           // RESTART: direction out
+  void *state = nullptr;       // device SolverState updated by the last block
+  int scalar_op = 0;           // ScalarOp run by the last block
+  const int *gate = nullptr;   // kernel is a no-op when *gate == 0
+};
+
+template <typename T>
+void launch_dslash(const LinksT<T> &L, int cb, T *out, const T *in, int epi, const DslashExtra &ex);
+
+void launch_dslash_half(const LinksT<float> &L, int cb, HalfSite *out_h, float *out_f,
+                        const HalfSite *in, int epi, const DslashExtra &ex);
+
+// FP32 copy of FP64 links in the same layout (fields.hip)
+void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf);
+
+} // namespace qphix

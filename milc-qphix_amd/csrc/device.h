@@ -1,0 +1,194 @@
+// device.h -- device-side helpers: complex arithmetic, spinor and link access,
+// lattice neighbour arithmetic.
+#pragma once
+#include "common.h"
+
+namespace qphix {
+
+template <typename C> struct cplx {
+  C re, im;
+};
+
+template <typename C> __device__ __forceinline__ void cmac(cplx<C> &acc, cplx<C> a, cplx<C> b) {
+  acc.re = fma(a.re, b.re, acc.re);
+  acc.re = fma(-a.im, b.im, acc.re);
+  acc.im = fma(a.re, b.im, acc.im);
+  acc.im = fma(a.im, b.re, acc.im);
+}
+
+// acc += U v for U given row-major as 9 complex numbers
+template <typename C>
+__device__ __forceinline__ void matvec_acc(const cplx<C> *U, const cplx<C> *v, cplx<C> *acc) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cmac(acc[r], U[3 * r + c], v[c]);
+  }
+}
+
+// Third row of a link from its first two: s * conj(r0 x r1).
+template <typename C>
+__device__ __forceinline__ void recon_row2(cplx<C> *U, C s) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+    // (r0 x r1)_c = r0[c1] r1[c2] - r0[c2] r1[c1]
+    C re = U[c1].re * U[3 + c2].re - U[c1].im * U[3 + c2].im - U[c2].re * U[3 + c1].re +
+           U[c2].im * U[3 + c1].im;
+    C im = U[c1].re * U[3 + c2].im + U[c1].im * U[3 + c2].re - U[c2].re * U[3 + c1].im -
+           U[c2].im * U[3 + c1].re;
+    U[6 + c].re = s * re;
+    U[6 + c].im = -s * im;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spinor (colour vector) storage traits.  Compute type is double for double
+// storage and float otherwise.
+// ---------------------------------------------------------------------------
+template <typename S> struct Spin;
+
+template <> struct Spin<double> {
+  using C = double;
+  static __device__ __forceinline__ void load(const double *p, int i, cplx<double> *v) {
+    const double2 *q = reinterpret_cast<const double2 *>(p + 6 * (size_t)i);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double2 d = q[c];
+      v[c].re = d.x;
+      v[c].im = d.y;
+    }
+  }
+  static __device__ __forceinline__ void store(double *p, int i, const cplx<double> *v) {
+    double2 *q = reinterpret_cast<double2 *>(p + 6 * (size_t)i);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[c] = make_double2(v[c].re, v[c].im);
+  }
+};
+
+template <> struct Spin<float> {
+  using C = float;
+  static __device__ __forceinline__ void load(const float *p, int i, cplx<float> *v) {
+    const float2 *q = reinterpret_cast<const float2 *>(p + 6 * (size_t)i);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float2 d = q[c];
+      v[c].re = d.x;
+      v[c].im = d.y;
+    }
+  }
+  static __device__ __forceinline__ void store(float *p, int i, const cplx<float> *v) {
+    float2 *q = reinterpret_cast<float2 *>(p + 6 * (size_t)i);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[c] = make_float2(v[c].re, v[c].im);
+  }
+};
+
+template <> struct Spin<HalfSite> {
+  using C = float;
+  static __device__ __forceinline__ void load(const HalfSite *p, int i, cplx<float> *v) {
+    const uint4 raw = *reinterpret_cast<const uint4 *>(p + i);
+    const __half2 h0 = *reinterpret_cast<const __half2 *>(&raw.x);
+    const __half2 h1 = *reinterpret_cast<const __half2 *>(&raw.y);
+    const __half2 h2 = *reinterpret_cast<const __half2 *>(&raw.z);
+    const float nrm = __uint_as_float(raw.w);
+    const float2 f0 = __half22float2(h0), f1 = __half22float2(h1), f2 = __half22float2(h2);
+    v[0].re = f0.x * nrm; v[0].im = f0.y * nrm;
+    v[1].re = f1.x * nrm; v[1].im = f1.y * nrm;
+    v[2].re = f2.x * nrm; v[2].im = f2.y * nrm;
+  }
+  static __device__ __forceinline__ void store(HalfSite *p, int i, const cplx<float> *v) {
+    float m = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v[c].re), fabsf(v[c].im)));
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+    uint4 raw;
+    __half2 h0 = __floats2half2_rn(v[0].re * inv, v[0].im * inv);
+    __half2 h1 = __floats2half2_rn(v[1].re * inv, v[1].im * inv);
+    __half2 h2 = __floats2half2_rn(v[2].re * inv, v[2].im * inv);
+    raw.x = *reinterpret_cast<unsigned *>(&h0);
+    raw.y = *reinterpret_cast<unsigned *>(&h1);
+    raw.z = *reinterpret_cast<unsigned *>(&h2);
+    raw.w = __float_as_uint(m);
+    *reinterpret_cast<uint4 *>(p + i) = raw;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Link pair loads from the chunk-major layout (see common.h).
+// NC complex numbers per pair; chunk = 16 bytes.
+// ---------------------------------------------------------------------------
+template <typename T, int NC> struct PairLoad;
+
+template <int NC> struct PairLoad<double, NC> {
+  static __device__ __forceinline__ void load(const double2 *base, int vh, int i, cplx<double> *u) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const double2 d = base[(size_t)k * vh + i];
+      u[k].re = d.x;
+      u[k].im = d.y;
+    }
+  }
+};
+
+template <int NC> struct PairLoad<float, NC> {
+  static __device__ __forceinline__ void load(const float4 *base, int vh, int i, cplx<float> *u) {
+#pragma unroll
+    for (int k = 0; k < NC / 2; ++k) {
+      const float4 d = base[(size_t)k * vh + i];
+      u[2 * k].re = d.x;
+      u[2 * k].im = d.y;
+      u[2 * k + 1].re = d.z;
+      u[2 * k + 1].im = d.w;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Lattice arithmetic.  cb = parity of the site i; the returned indices are
+// raw indices of the opposite parity.
+// ---------------------------------------------------------------------------
+struct Geo {
+  int nxh, ny, nz, nt, vh, nxy, nxyz;
+};
+
+struct SiteNbr {
+  int f1[4], b1[4], f3[4], b3[4];
+};
+
+__device__ __forceinline__ void site_coords(const Geo &g, int i, int &xh, int &y, int &z, int &t) {
+  xh = i % g.nxh;
+  int r = i / g.nxh;
+  y = r % g.ny;
+  r /= g.ny;
+  z = r % g.nz;
+  t = r / g.nz;
+}
+
+__device__ __forceinline__ int wrapc(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+__device__ __forceinline__ void site_neighbours(const Geo &g, int i, int cb, SiteNbr &n) {
+  int xh, y, z, t;
+  site_coords(g, i, xh, y, z, t);
+  const int xodd = (y + z + t + cb) & 1;
+  const int row = i - xh;
+  n.f1[0] = row + wrapc(xh + xodd, g.nxh);
+  n.b1[0] = row + wrapc(xh + xodd - 1, g.nxh);
+  n.f3[0] = row + wrapc(xh + xodd + 1, g.nxh);
+  n.b3[0] = row + wrapc(xh + xodd - 2, g.nxh);
+  // y, z, t: single wrap suffices because every extent is >= 4
+  const int cs[3] = {y, z, t};
+  const int len[3] = {g.ny, g.nz, g.nt};
+  const int st[3] = {g.nxh, g.nxy, g.nxyz};
+  const int per[3] = {g.nxy, g.nxyz, g.vh};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int c = cs[d], L = len[d], s = st[d], P = per[d];
+    n.f1[d + 1] = i + s - (c + 1 >= L ? P : 0);
+    n.b1[d + 1] = i - s + (c - 1 < 0 ? P : 0);
+    n.f3[d + 1] = i + 3 * s - (c + 3 >= L ? P : 0);
+    n.b3[d + 1] = i - 3 * s + (c - 3 < 0 ? P : 0);
+  }
+}
+
+} // namespace qphix

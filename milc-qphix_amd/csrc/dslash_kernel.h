@@ -1,0 +1,118 @@
+// dslash_kernel.h -- the Dslash kernel template (one work-item per output site).
+#pragma once
+#include "reduce.h"
+
+namespace qphix {
+
+constexpr int DSLASH_BLOCK = 256;
+
+template <typename TL, typename SI, typename SO> struct DslashKArgs {
+  const typename vec16<TL>::type *fat;  // output parity's links, chunk-major
+  const typename vec16<TL>::type *lng;
+  const uint8_t *lsign;                 // 12-recon signs (bit 2mu+1 fwd, 2mu back)
+  const SI *in;
+  SO *out;
+  Geo g;
+  int cb;
+  TL s;         // epilogue scalar
+  const SI *aux;
+  const SI *b;
+  SO *r, *p;
+  ReduceCtl rc;
+  const int *gate;
+};
+
+// Applies one direction's link pair (forward link on in(fwd), back link on
+// in(bwd)).  NC = 18: both links complete; NC = 12: rows 0-1 of each, row 2
+// rebuilt as sign * conj(r0 x r1).
+template <typename TL, int NC, typename SI>
+__device__ __forceinline__ void apply_pair(const typename vec16<TL>::type *base, int vh, int i,
+                                           const SI *in, int nf, int nb, unsigned sg_f,
+                                           unsigned sg_b, cplx<TL> *acc) {
+  cplx<TL> u[NC];
+  PairLoad<TL, NC>::load(base, vh, i, u);
+  cplx<TL> vf[3], vb[3];
+  Spin<SI>::load(in, nf, vf);
+  Spin<SI>::load(in, nb, vb);
+  if constexpr (NC == 18) {
+    matvec_acc(u, vf, acc);
+    matvec_acc(u + 9, vb, acc);
+  } else {
+    cplx<TL> w[9];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = u[k];
+    recon_row2(w, sg_f ? (TL)-1 : (TL)1);
+    matvec_acc(w, vf, acc);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = u[6 + k];
+    recon_row2(w, sg_b ? (TL)-1 : (TL)1);
+    matvec_acc(w, vb, acc);
+  }
+}
+
+template <typename TL, int NL, typename SI, typename SO, int EPI>
+__global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash(const DslashKArgs<TL, SI, SO> a) {
+  if (a.gate && *a.gate == 0) return;  // uniform over the grid
+  constexpr int CPV = vec16<TL>::ncplx;
+  constexpr int NCH_F = 18 / CPV, NCH_L = NL / CPV;
+  const int vh = a.g.vh;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < vh;
+  cplx<TL> acc[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) acc[c] = cplx<TL>{0, 0};
+  if (valid) {
+    SiteNbr n;
+    site_neighbours(a.g, i, a.cb, n);
+    unsigned sg = 0;
+    if constexpr (NL == 12) sg = a.lsign[i];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      apply_pair<TL, 18, SI>(a.fat + (size_t)mu * NCH_F * vh, vh, i, a.in, n.f1[mu], n.b1[mu], 0,
+                             0, acc);
+      apply_pair<TL, NL, SI>(a.lng + (size_t)mu * NCH_L * vh, vh, i, a.in, n.f3[mu], n.b3[mu],
+                             (sg >> (2 * mu + 1)) & 1u, (sg >> (2 * mu)) & 1u, acc);
+    }
+  }
+  if constexpr (EPI == EPI_PLAIN) {
+    if (valid) Spin<SO>::store(a.out, i, acc);
+  } else if constexpr (EPI == EPI_PKP) {
+    double v[1] = {0.0};
+    if (valid) {
+      cplx<TL> p[3];
+      Spin<SI>::load(a.aux, i, p);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        acc[c].re = acc[c].re + a.s * p[c].re;
+        acc[c].im = acc[c].im + a.s * p[c].im;
+        v[0] += (double)p[c].re * (double)acc[c].re + (double)p[c].im * (double)acc[c].im;
+      }
+      Spin<SO>::store(a.out, i, acc);
+    }
+    grid_reduce<1>(v, a.rc);
+  } else {  // EPI_RESTART
+    double v[2] = {0.0, 0.0};
+    if (valid) {
+      cplx<TL> x[3], b[3];
+      Spin<SI>::load(a.aux, i, x);
+      Spin<SI>::load(a.b, i, b);
+      double num = 0, den = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const TL tr = acc[c].re + a.s * x[c].re;
+        const TL ti = acc[c].im + a.s * x[c].im;
+        acc[c].re = b[c].re + tr;
+        acc[c].im = b[c].im + ti;
+        num += (double)acc[c].re * acc[c].re + (double)acc[c].im * acc[c].im;
+        den += (double)x[c].re * x[c].re + (double)x[c].im * x[c].im;
+      }
+      Spin<SO>::store(a.r, i, acc);
+      Spin<SO>::store(a.p, i, acc);
+      v[0] = num;
+      v[1] = num == 0 ? 0.0 : (den == 0 ? 1.0 : num / den);
+    }
+    grid_reduce<2>(v, a.rc);
+  }
+}
+
+} // namespace qphix

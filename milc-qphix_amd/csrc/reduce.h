@@ -1,0 +1,236 @@
+// reduce.h -- deterministic grid reductions and the on-device solver scalars.
+//
+// Every reducing kernel writes one partial per block; the last block to
+// finish (atomic counter) sums the partials in a fixed order and runs the
+// solver's scalar update in place.  The reference does the same arithmetic on
+// the host after an OpenMP reduction + g_doublesum (ks_d_congrad_fn.cpp:486-566,
+// ks_multicg_offset.cpp:463-598); keeping it on the device lets the host queue
+// many iterations without reading anything back.
+#pragma once
+#include "device.h"
+
+namespace qphix {
+
+constexpr int MAXNDMASS = 150;  // ks_d_congrad_fn.h:4
+
+struct SolverState {
+  double rsq, oldrsq, pkp, relrsq, src_norm;
+  double size_r, size_relr, final_rsq, final_rel;
+  double a, b, msq_x4, rsqmin, relrsqmin, rsqstop, vol;
+  int iteration, niter, max_cg, active;
+  int single_coef;  // 1: scalars rounded to float as the F build's fptype does
+  int converged, nshift, j_low, num_now, pad_;
+  double offset_low;
+  double shifts[MAXNDMASS], zeta_i[MAXNDMASS], zeta_im1[MAXNDMASS], zeta_ip1[MAXNDMASS];
+  double beta_i[MAXNDMASS], beta_im1[MAXNDMASS], alpha[MAXNDMASS];
+  int finished[MAXNDMASS];
+  // mixed-precision solver scalars
+  double mx_alpha, mx_beta, mx_rsq, mx_maxr, mx_r0, mx_pAp;
+  int mx_update, mx_pad_;
+};
+
+enum ScalarOp {
+  SOP_NONE = 0,
+  SOP_SRC_NORM,   // src_norm = v0
+  SOP_CG_PKP,     // ks_d_congrad_fn.cpp:486-511
+  SOP_CG_RSQ2,    // ks_d_congrad_fn.cpp:524-572
+  SOP_CG_RESTART, // ks_d_congrad_fn.cpp:372-419
+  SOP_MS_PKP,     // ks_multicg_offset.cpp:463-496
+  SOP_MS_RSQ2,    // ks_multicg_offset.cpp:504-598
+  SOP_STORE_RSQ,  // rsq = v0
+  SOP_MX_PAP,     // mixed CG: alpha = rsq / pAp
+  SOP_MX_RSQ      // mixed CG: new |r|^2, reliable-update test, beta
+};
+
+// (fptype) cast of the reference: round to float when the solver is the F build.
+__device__ __forceinline__ double rnd(const SolverState *s, double v) {
+  return s->single_coef ? (double)(float)v : v;
+}
+
+template <typename R> __device__ void ms_pkp_recurrence(SolverState *s) {
+  const int jl = s->j_low;
+  s->beta_i[jl] = (double)(R)(-s->rsq / s->pkp);
+  s->zeta_ip1[jl] = 1.0;
+  for (int j = 0; j < s->num_now; ++j) {
+    if (j == jl) continue;
+    R zi = (R)s->zeta_i[j], zim1 = (R)s->zeta_im1[j], bim1 = (R)s->beta_im1[jl];
+    R bl = (R)s->beta_i[jl], al = (R)s->alpha[jl], sh = (R)s->shifts[j];
+    R zip1 = zi * zim1 * bim1;
+    const R c1 = bl * al * (zim1 - zi);
+    const R c2 = (R)(zim1 * bim1 * (1.0 + sh * bl));
+    R bj;
+    if (c1 + c2 != (R)0.0) zip1 /= c1 + c2;
+    else { zip1 = 0; s->finished[j] = 1; }
+    if (zi != (R)0.0) bj = bl * zip1 / zi;
+    else { zip1 = 0; bj = 0; s->finished[j] = 1; }
+    s->zeta_ip1[j] = zip1;
+    s->beta_i[j] = bj;
+    if (s->finished[j] == 1 && j == s->num_now - 1 && j > jl) s->num_now--;
+  }
+}
+
+template <typename R> __device__ void ms_alpha_recurrence(SolverState *s) {
+  const int jl = s->j_low;
+  s->alpha[jl] = (double)(R)(s->rsq / s->oldrsq);
+  for (int j = 0; j < s->num_now; ++j) {
+    if (j == jl) continue;
+    const R zi = (R)s->zeta_i[j], bl = (R)s->beta_i[jl];
+    if (zi * bl != (R)0.0)
+      s->alpha[j] = (R)s->alpha[jl] * (R)s->zeta_ip1[j] * (R)s->beta_i[j] / (zi * bl);
+    else { s->alpha[j] = 0; s->finished[j] = 1; }
+  }
+  for (int j = 0; j < s->num_now; ++j) {
+    s->beta_im1[j] = s->beta_i[j];
+    s->zeta_im1[j] = s->zeta_i[j];
+    s->zeta_i[j] = s->zeta_ip1[j];
+  }
+}
+
+// Runs in one thread of the last block.  v[0], v[1] are the reduced sums.
+__device__ inline void apply_scalar_op(SolverState *s, int op, const double *v) {
+  switch (op) {
+  case SOP_SRC_NORM:
+    s->src_norm = v[0];
+    break;
+  case SOP_STORE_RSQ:
+    s->rsq = v[0];
+    break;
+  case SOP_CG_PKP:
+    s->oldrsq = s->rsq;
+    s->pkp = v[0];
+    s->iteration++;
+    s->a = rnd(s, -s->rsq / s->pkp);
+    break;
+  case SOP_CG_RSQ2: {
+    s->rsq = v[0];
+    if (s->relrsqmin > 0) s->relrsq = rnd(s, sqrt(2.0 * v[1] / s->vol));
+    s->size_r = rnd(s, s->rsq) / s->src_norm;
+    s->size_relr = rnd(s, s->relrsq);
+    s->final_rsq = s->size_r;
+    s->b = rnd(s, rnd(s, s->rsq) / s->oldrsq);
+    const bool conv = (s->rsqmin <= 0 || s->rsqmin > s->size_r) &&
+                      (s->relrsqmin <= 0 || s->relrsqmin > s->size_relr);
+    if (s->iteration % s->niter == 0 || conv) s->active = 0;
+    break;
+  }
+  case SOP_CG_RESTART:
+    s->rsq = v[0];
+    if (s->relrsqmin > 0) s->relrsq = rnd(s, sqrt(2.0 * v[1] / s->vol));
+    s->final_rsq = rnd(s, s->rsq) / s->src_norm;
+    s->final_rel = rnd(s, s->relrsq);
+    s->iteration++;
+    break;
+  case SOP_MS_PKP:
+    s->oldrsq = s->rsq;
+    s->pkp = v[0];
+    s->iteration++;
+    if (s->single_coef) ms_pkp_recurrence<float>(s);
+    else ms_pkp_recurrence<double>(s);
+    break;
+  case SOP_MS_RSQ2:
+    s->rsq = v[0];
+    if (s->rsq <= s->rsqstop) {
+      s->converged = 1;
+      s->active = 0;
+      break;
+    }
+    // The host never queues more than max_cg steps, so no cap is needed here.
+    if (s->single_coef) ms_alpha_recurrence<float>(s);
+    else ms_alpha_recurrence<double>(s);
+    break;
+  case SOP_MX_PAP:
+    s->mx_pAp = v[0];
+    s->iteration++;
+    s->mx_alpha = s->mx_rsq / v[0];
+    break;
+  case SOP_MX_RSQ: {
+    // Reliable-update test (Clark et al., arXiv:0911.3191, sec. IV): flag an
+    // update when |r| has dropped by `mx_r0` (delta) since the last one.
+    const double rnew = v[0];
+    s->mx_beta = rnew / s->mx_rsq;
+    s->mx_rsq = rnew;
+    const double r = sqrt(rnew);
+    if (r > s->mx_maxr) s->mx_maxr = r;
+    const bool conv = rnew < s->rsqstop;
+    if (r < s->mx_r0 * s->mx_maxr || conv || s->iteration % s->niter == 0) {
+      s->mx_update = 1;
+      s->active = 0;
+    }
+    if (s->iteration >= s->max_cg) s->active = 0;
+    break;
+  }
+  default:
+    break;
+  }
+}
+
+// Block-wide sum of NV values; result valid in thread 0.  blockDim.x <= 1024.
+template <int NV> __device__ __forceinline__ void block_sum(double *v) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  }
+  __shared__ double sh[NV][16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0;
+      for (int j = 0; j < nw; ++j) s += sh[k][j];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+struct ReduceCtl {
+  double *partials;
+  unsigned *counter;
+  SolverState *state;  // scalar update target, or null
+  int op;
+  double *out;         // when state is null: the NV sums are stored here
+};
+
+// Every thread of every block calls this once with its contribution.
+template <int NV> __device__ void grid_reduce(double *v, const ReduceCtl &rc) {
+  block_sum<NV>(v);
+  __shared__ int am_last;
+  const int nb = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) rc.partials[(size_t)k * nb + blockIdx.x] = v[k];
+    __threadfence();
+    const unsigned prev = atomicAdd(rc.counter, 1u);
+    am_last = (prev == (unsigned)nb - 1);
+  }
+  __syncthreads();
+  if (!am_last) return;
+  __threadfence();
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    s[k] = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+      s[k] += __hip_atomic_load(rc.partials + (size_t)k * nb + b, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum<NV>(s);
+  if (threadIdx.x == 0) {
+    if (rc.state) apply_scalar_op(rc.state, rc.op, s);
+    else if (rc.out) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) rc.out[k] = s[k];
+    }
+    __hip_atomic_store(rc.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+} // namespace qphix

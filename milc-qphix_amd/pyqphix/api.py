@@ -1,0 +1,124 @@
+"""Object wrapper over the C interface (ctypes), for tests and benchmarks.
+
+Mirrors the reference's API one to one: Vector = QPHIX_*3_ColorVector,
+Links = QPHIX_*3_FermionLinksAsqtad, Gauge = QPHIX_*3_GaugeField,
+Force = QPHIX_*3_Force; dslash / invert / invert_multi /
+symanzik_1loop_gauge_force call the functions of the same name.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import lib as _L
+from .lib import EVEN, ODD, EVENODD, Info, InvertArg, ResidArg, GaugeCoeffs, ptr
+
+_NP = {"D": np.float64, "F": np.float32}
+_CR = {"D": C.c_double, "F": C.c_float}
+
+
+class Session:
+    """QPHIX_init .. QPHIX_finalize for one lattice."""
+
+    def __init__(self, dims, lib=None):
+        self.lib = lib or _L.load()
+        self.dims = tuple(dims)
+        self.V = int(np.prod(dims))
+        self.vh = self.V // 2
+        st = self.lib.init(self.dims)
+        if st != 0:
+            n, msg = self.lib.last_error()
+            raise RuntimeError(f"QPHIX_init failed: {msg}")
+
+    def close(self):
+        self.lib.QPHIX_finalize()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def fn(self, prec, name):
+        return getattr(self.lib, f"QPHIX_{prec}3_{name}")
+
+
+class Vector:
+    def __init__(self, s, raw, parity, prec="D"):
+        self.s, self.prec, self.parity = s, prec, parity
+        if raw is None:
+            self._src = None
+            self.h = s.fn(prec, "create_V_from_raw")(None, parity)
+        else:
+            self._src = np.ascontiguousarray(raw, dtype=_NP[prec])
+            assert self._src.size == 6 * s.V
+            self.h = s.fn(prec, "create_V_from_raw")(ptr(self._src), parity)
+        if not self.h:
+            raise RuntimeError("create_V failed: %s" % (s.lib.last_error(),))
+
+    def extract(self):
+        out = np.zeros(6 * self.s.V, dtype=_NP[self.prec])
+        self.s.fn(self.prec, "extract_V_to_raw")(ptr(out), self.h, self.parity)
+        return out
+
+    def free(self):
+        if self.h:
+            self.s.fn(self.prec, "destroy_V")(self.h)
+            self.h = None
+
+
+class Links:
+    def __init__(self, s, fat, lng, fatback, lngback, parity=EVENODD, prec="D"):
+        self.s, self.prec, self.parity = s, prec, parity
+        arrs = [np.ascontiguousarray(a, dtype=_NP[prec]) for a in (fat, lng, fatback, lngback)]
+        self.h = s.fn(prec, "asqtad_create_L_from_raw")(*[ptr(a) for a in arrs], parity)
+        if not self.h:
+            raise RuntimeError("create_L failed: %s" % (s.lib.last_error(),))
+
+    @property
+    def recon(self):
+        return self.s.fn(self.prec, "asqtad_links_recon")(self.h)
+
+    def extract(self, parity=EVENODD):
+        fat = np.zeros(72 * self.s.V, dtype=_NP[self.prec])
+        lng = np.zeros_like(fat)
+        self.s.fn(self.prec, "asqtad_extract_L_to_raw")(ptr(fat), ptr(lng), self.h, parity)
+        return fat, lng
+
+    def backcheck(self):
+        out = (C.c_double * 4)()
+        self.s.lib.QPHIX_D3_asqtad_links_backcheck(self.h, out)
+        return list(out)
+
+    def free(self):
+        if self.h:
+            self.s.fn(self.prec, "asqtad_destroy_L")(self.h)
+            self.h = None
+
+
+def dslash(L, out, inp, parity):
+    L.s.fn(L.prec, "asqtad_dslash")(L.h, out.h, inp.h, parity)
+
+
+def dslash_time(L, out, inp, parity, reps):
+    return L.s.fn(L.prec, "asqtad_dslash_time")(L.h, out.h, inp.h, parity, reps)
+
+
+def invert(L, out, inp, mass, parity, resid, relresid=0.0, niter=1000, nrestart=5):
+    info = Info()
+    inv = InvertArg(parity, niter, 0, nrestart, parity)
+    res = ResidArg(resid, 0, relresid, 0, 0, 0, 0, 0)
+    ret = L.s.fn(L.prec, "asqtad_invert")(C.byref(info), L.h, C.byref(inv), C.byref(res),
+                                          mass, out.h, inp.h)
+    return ret, info, res
+
+
+def invert_multi(L, outs, inp, masses, parity, resid, niter=1000, nrestart=5):
+    info = Info()
+    inv = InvertArg(parity, niter, 0, nrestart, parity)
+    res = [ResidArg(resid, 0, 0, 0, 0, 0, 0, 0) for _ in masses]
+    resp = (C.POINTER(ResidArg) * len(masses))(*[C.pointer(r) for r in res])
+    m = (_CR[L.prec] * len(masses))(*masses)
+    hs = (C.c_void_p * len(masses))(*[o.h for o in outs])
+    ret = L.s.fn(L.prec, "asqtad_invert_multi")(C.byref(info), L.h, C.byref(inv), resp, m,
+                                                len(masses), hs, inp.h)
+    return ret, info, res

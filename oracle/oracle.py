@@ -1,0 +1,121 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so, built from qphix_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline leg of bench.py, always as the checker or the baseline, never as a
+compute path of the library.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+
+EVEN, ODD, EVENODD = 2, 1, 3
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        L.oracle_setup.restype = C.c_int
+        L.oracle_setup.argtypes = [C.POINTER(C.c_int)]
+        L.oracle_neighbours.argtypes = [C.c_int, vp]
+        L.oracle_num_threads.restype = C.c_int
+        for p in ("d", "f"):
+            getattr(L, f"oracle_links_create_{p}").restype = vp
+            getattr(L, f"oracle_links_create_{p}").argtypes = [vp, vp, vp, vp]
+            getattr(L, f"oracle_links_destroy_{p}").argtypes = [vp]
+            getattr(L, f"oracle_dslash_{p}").argtypes = [vp, vp, vp, C.c_int]
+            R = C.c_double if p == "d" else C.c_float
+            getattr(L, f"oracle_invert_{p}").restype = C.c_int
+            getattr(L, f"oracle_invert_{p}").argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                         C.c_double, R, vp, vp, vp, vp]
+            getattr(L, f"oracle_invert_multi_{p}").restype = C.c_int
+            getattr(L, f"oracle_invert_multi_{p}").argtypes = [vp, C.c_int, C.c_int, C.c_int,
+                                                               C.c_double, vp, C.c_int, vp, vp, vp, vp]
+        L.oracle_relative_residue_d.restype = C.c_double
+        L.oracle_relative_residue_d.argtypes = [vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+class Oracle:
+    """One lattice's geometry plus one link set, in one precision."""
+
+    def __init__(self, dims, fat, lng, fatback, lngback, dtype=np.float64):
+        self.L = lib()
+        self.dims = tuple(dims)
+        d = (C.c_int * 4)(*dims)
+        if self.L.oracle_setup(d) != 0:
+            raise ValueError("bad lattice")
+        self.vh = int(np.prod(dims)) // 2
+        self.dtype = np.dtype(dtype)
+        self.p = "d" if self.dtype == np.float64 else "f"
+        self._arrs = [np.ascontiguousarray(a, dtype=self.dtype) for a in (fat, lng, fatback, lngback)]
+        self.h = getattr(self.L, f"oracle_links_create_{self.p}")(*[_p(a) for a in self._arrs])
+
+    def close(self):
+        if self.h:
+            getattr(self.L, f"oracle_links_destroy_{self.p}")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def neighbours(self, cb):
+        out = np.empty(16 * self.vh, dtype=np.uint32)
+        self.L.oracle_neighbours(cb, _p(out))
+        return out.reshape(self.vh, 16)
+
+    def dslash(self, inp_half, cb):
+        """out(half of parity cb) = D in(half of parity 1-cb)."""
+        inp = np.ascontiguousarray(inp_half, dtype=self.dtype)
+        out = np.empty(6 * self.vh, dtype=self.dtype)
+        getattr(self.L, f"oracle_dslash_{self.p}")(self.h, _p(out), _p(inp), cb)
+        return out
+
+    def invert(self, b_full, mass, parity, resid, relresid=0.0, niter=1000, nrestart=5, x0=None):
+        b = np.ascontiguousarray(b_full, dtype=self.dtype)
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=self.dtype).copy()
+        od = np.zeros(5)
+        oi = np.zeros(2, dtype=np.int32)
+        ret = getattr(self.L, f"oracle_invert_{self.p}")(self.h, parity, niter, nrestart, resid,
+                                                         relresid, mass, _p(x), _p(b), _p(od), _p(oi))
+        return x, dict(ret=ret, final_rsq=od[0], final_rel=od[1], size_r=od[2], size_relr=od[3],
+                       final_flop=od[4], final_iter=int(oi[0]), final_restart=int(oi[1]))
+
+    def invert_multi(self, b_full, masses, parity, resid, niter=1000, nrestart=5):
+        b = np.ascontiguousarray(b_full, dtype=self.dtype)
+        xs = [np.zeros_like(b) for _ in masses]
+        ptrs = (C.c_void_p * len(masses))(*[x.ctypes.data for x in xs])
+        m = np.ascontiguousarray(masses, dtype=self.dtype)
+        od = np.zeros(2)
+        oi = np.zeros(1, dtype=np.int32)
+        ret = getattr(self.L, f"oracle_invert_multi_{self.p}")(
+            self.h, parity, niter, nrestart, resid, _p(m), len(masses), C.cast(ptrs, C.c_void_p),
+            _p(b), _p(od), _p(oi))
+        return xs, dict(ret=ret, final_rsq=od[0], final_flop=od[1], final_iter=int(oi[0]))

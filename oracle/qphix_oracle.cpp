@@ -1,0 +1,491 @@
+/* qphix_oracle.cpp -- CPU oracle for the staggered hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this file; tests/, the smoke check and the CPU-baseline leg of the
+ * benchmark use it as the checker.  It restates, in plain scalar C++ with
+ * OpenMP site loops, what the reference computes for:
+ *
+ *   geometry + neighbour tables    qphix_misc.cpp:46-230, 494-509
+ *   raw <-> per-site link slots    qphix_d3.cpp:970-1085 (C4: slot 2mu back, 2mu+1 forward)
+ *   Dslash site loop               ks_long_dslash.cpp:564-668 (16 neighbour terms, summed)
+ *   CG                             ks_d_congrad_fn.cpp:62-188 (BLAS), 221-619 (driver)
+ *   multi-shift CG                 ks_multicg_offset.cpp:69-184 (BLAS), 205-689 (driver)
+ *   relative residue               ks_blas_utils_c.cpp:110-149
+ *
+ * Layout is the reference's VECLEN=1 layout without the XY_PAD site (padding
+ * never changes a result): site index ind = xh + Nxh*(y + Ny*(z + Nz*t)) within
+ * a parity, i.e. the raw MILC index of that parity.
+ *
+ * The reference's kernel body (milc-qphix-codegen, absent) would fix the
+ * rounding order of the 16-term sum; the oracle sums in the order
+ * mu = X,Y,Z,T and, within mu, fat fwd, fat back, long fwd, long back.
+ * Parity pins for every function here are the closed forms and invariants in
+ * tests/test_oracle.py (free-field spectrum, anti-Hermiticity, gauge
+ * covariance, plane-wave CG/multi-shift solutions, dense solves at 4^4).
+ *
+ * Deliberate differences from the reference, shared with the product:
+ *  - EVENODD solves the even half with the even operator, then the odd half
+ *    with the odd one.  The reference's drivers take parity = 3 & 1 = 1 for the
+ *    even half (ks_d_congrad_fn.cpp:285,240-249), i.e. the odd-checkerboard
+ *    operator on even-site data; that is not reproduced.
+ *  - multi-shift `finished[]` starts at 0 (the reference mallocs it without
+ *    initialising, ks_multicg_offset.cpp:332).
+ */
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+enum { P_EVEN = 2, P_ODD = 1, P_EVENODD = 3 };
+
+struct Geom {
+  int nx = 0, ny = 0, nz = 0, nt = 0, nxh = 0;
+  long vh = 0;                    /* sites per parity */
+  std::vector<unsigned> neigh[2]; /* [cb][16*ind + k], qphix_misc.cpp:88-221 */
+};
+Geom G;
+
+/* qphix_misc.cpp:88-221 restated with periodic (and double) wrap.  cb is the
+ * parity of the site the table belongs to (0 even, 1 odd). */
+void build_neighbours() {
+  const int Vxh = G.nxh, Vy = G.ny, Vz = G.nz, Vt = G.nt;
+  const long Pxy = (long)Vxh * Vy, Pxyz = Pxy * Vz;
+  auto wrap = [](long v, long n) { v %= n; return v < 0 ? v + n : v; };
+  for (int cb = 0; cb < 2; ++cb) {
+    G.neigh[cb].assign(16 * G.vh, 0);
+    for (int t = 0; t < Vt; ++t)
+      for (int z = 0; z < Vz; ++z)
+        for (int y = 0; y < Vy; ++y)
+          for (int x = 0; x < Vxh; ++x) {
+            const int xodd = (y + z + t + cb) & 1;
+            const long ind = t * Pxyz + z * Pxy + (long)y * Vxh + x;
+            const long row = ind - x;
+            unsigned *n = &G.neigh[cb][16 * ind];
+            const int dx1[2] = {xodd - 1, xodd};
+            const int dx3[2] = {xodd - 2, xodd + 1};
+            for (int fb = 0; fb < 2; ++fb) {
+              n[0 + fb] = (unsigned)(row + wrap(x + dx1[fb], Vxh));
+              n[8 + fb] = (unsigned)(row + wrap(x + dx3[fb], Vxh));
+            }
+            const int dir_len[3] = {Vy, Vz, Vt};
+            const int coord[3] = {y, z, t};
+            const long stride[3] = {Vxh, Pxy, Pxyz};
+            for (int d = 0; d < 3; ++d) {
+              const long base = ind - coord[d] * stride[d];
+              for (int fb = 0; fb < 2; ++fb) {
+                const int s = fb ? 1 : -1;
+                n[2 + 2 * d + fb] = (unsigned)(base + wrap(coord[d] + s, dir_len[d]) * stride[d]);
+                n[10 + 2 * d + fb] = (unsigned)(base + wrap(coord[d] + 3 * s, dir_len[d]) * stride[d]);
+              }
+            }
+          }
+  }
+}
+
+/* Links of one output parity in the reference's per-site slot order:
+ * [ind][slot 0..7][row][col][re,im] for fat and long (qcd_data_types.h:36-44). */
+template <typename T> struct Links {
+  std::vector<T> fat[2], lng[2];
+};
+
+template <typename T>
+Links<T> *links_create(const T *fat, const T *lng, const T *fatback, const T *lngback) {
+  auto *L = new Links<T>;
+  for (int cb = 0; cb < 2; ++cb) {
+    L->fat[cb].assign(144 * G.vh, 0);
+    L->lng[cb].assign(144 * G.vh, 0);
+    const long off = cb ? 72 * G.vh : 0; /* raw odd block at 72*even_sites */
+#pragma omp parallel for
+    for (long i = 0; i < G.vh; ++i)
+      for (int mu = 0; mu < 4; ++mu)
+        for (int e = 0; e < 18; ++e) {
+          const long src = off + 72 * i + 18 * mu + e;
+          L->fat[cb][144 * i + 18 * (2 * mu + 1) + e] = fat[src];
+          L->fat[cb][144 * i + 18 * (2 * mu) + e] = fatback[src];
+          L->lng[cb][144 * i + 18 * (2 * mu + 1) + e] = lng[src];
+          L->lng[cb][144 * i + 18 * (2 * mu) + e] = lngback[src];
+        }
+  }
+  return L;
+}
+
+/* acc += U * v, U 3x3 complex row-major [r][c][re,im], v [c][re,im] */
+template <typename T> inline void mat_vec_acc(const T *U, const T *v, T *acc) {
+  for (int r = 0; r < 3; ++r) {
+    T re = 0, im = 0;
+    for (int c = 0; c < 3; ++c) {
+      const T ur = U[6 * r + 2 * c], ui = U[6 * r + 2 * c + 1];
+      re += ur * v[2 * c] - ui * v[2 * c + 1];
+      im += ur * v[2 * c + 1] + ui * v[2 * c];
+    }
+    acc[2 * r] += re;
+    acc[2 * r + 1] += im;
+  }
+}
+
+/* ks_long_dslash.cpp:564-668: for every site of output parity cb,
+ * out = sum_k slot_k * in[neigh_k].  `in` and `out` are half-lattice arrays
+ * of 6 reals per site. */
+template <typename T> void dslash(const Links<T> *L, T *out, const T *in, int cb) {
+  const unsigned *nt = G.neigh[cb].data();
+  const T *gf = L->fat[cb].data();
+  const T *gl = L->lng[cb].data();
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < G.vh; ++i) {
+    T acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int mu = 0; mu < 4; ++mu) {
+      mat_vec_acc(gf + 144 * i + 18 * (2 * mu + 1), in + 6 * (long)nt[16 * i + 2 * mu + 1], acc);
+      mat_vec_acc(gf + 144 * i + 18 * (2 * mu), in + 6 * (long)nt[16 * i + 2 * mu], acc);
+      mat_vec_acc(gl + 144 * i + 18 * (2 * mu + 1), in + 6 * (long)nt[16 * i + 8 + 2 * mu + 1], acc);
+      mat_vec_acc(gl + 144 * i + 18 * (2 * mu), in + 6 * (long)nt[16 * i + 8 + 2 * mu], acc);
+    }
+    for (int e = 0; e < 6; ++e) out[6 * i + e] = acc[e];
+  }
+}
+
+/* Reductions accumulate in the vector type T, as the reference's OpenMP
+ * reductions do (`fptype rsq = 0.0`, ks_d_congrad_fn.cpp:65,90,139,163), in a
+ * fixed blocked order so results do not depend on the thread count. */
+template <typename T, typename F> double blocked_sum(long n, F term) {
+  const long B = 4096;
+  const long nb = (n + B - 1) / B;
+  std::vector<T> part(nb);
+#pragma omp parallel for schedule(static)
+  for (long b = 0; b < nb; ++b) {
+    T s = 0;
+    const long e = (b + 1) * B < n ? (b + 1) * B : n;
+    for (long i = b * B; i < e; ++i) s += term(i);
+    part[b] = s;
+  }
+  T tot = 0;
+  for (long b = 0; b < nb; ++b) tot += part[b];
+  return (double)tot;
+}
+
+/* ks_blas_utils_c.cpp:110-149 */
+template <typename T> double relative_residue(const T *r, const T *x) {
+  double res = 0;
+  for (long i = 0; i < G.vh; ++i) {
+    double num = 0, den = 0;
+    for (int e = 0; e < 6; ++e) {
+      num += (double)r[6 * i + e] * r[6 * i + e];
+      den += (double)x[6 * i + e] * x[6 * i + e];
+    }
+    if (num == 0) continue;
+    res += den == 0 ? 1.0 : num / den;
+  }
+  return std::sqrt(2 * res / ((double)G.nx * G.ny * G.nz * G.nt));
+}
+
+struct InvertOut {
+  double final_rsq, final_rel, size_r, size_relr, final_flop;
+  int final_iter, final_restart, ret;
+};
+
+/* ks_d_congrad_fn.cpp:221-619 restated.  `x` and `b` are full raw arrays
+ * (even block then odd block); parity is the API parity. */
+template <typename T>
+void invert(const Links<T> *L, int parity_api, int niter, int max_restarts, double resid,
+            double relresid, T mass, T *x_full, const T *b_full, InvertOut *o) {
+  const long n = 6 * G.vh;
+  std::vector<T> ttt(n), tmp(n), p(n), r(n);
+  const T rsqmin = (T)resid, relrsqmin = (T)relresid;
+  const int max_cg = max_restarts * niter;
+  const T msq_x4 = (T)(4.0 * mass * mass);
+  int total_iters = 0, iteration = 0, nrestart = 0;
+  double rsq = 0, relrsq = 0, oldrsq, pkp;
+  double size_r = 0, size_relr = 0, final_rsq = 0, final_rel = 0;
+  int todo[2], ntodo = 0;
+  if (parity_api == P_EVENODD) { todo[0] = 0; todo[1] = 1; ntodo = 2; }
+  else { todo[0] = parity_api == P_EVEN ? 0 : 1; ntodo = 1; }
+  for (int pass = 0; pass < ntodo; ++pass) {
+    const int parity = todo[pass], other = 1 - parity;
+    T *xd = x_full + (parity ? n : 0);
+    const T *bs = b_full + (parity ? n : 0);
+    const double source_norm = blocked_sum<T>(n, [&](long i) { return bs[i] * bs[i]; });
+    nrestart = 0;
+    total_iters += iteration;
+    iteration = 0;
+    size_r = 0;
+    size_relr = 0;
+    if (source_norm == 0.0) {
+      for (long i = 0; i < n; ++i) xd[i] = 0;
+      final_rel = 0;
+      final_rsq = 0;
+      continue;
+    }
+    while (true) {
+      if ((iteration % niter == 0) ||
+          ((rsqmin <= 0 || rsqmin > size_r) && (relrsqmin <= 0 || relrsqmin > size_relr))) {
+        dslash(L, tmp.data(), xd, other);
+        dslash(L, ttt.data(), tmp.data(), parity);
+        /* calc_rsq, ks_d_congrad_fn.cpp:82-99 */
+        rsq = blocked_sum<T>(n, [&](long i) {
+          ttt[i] = ttt[i] + (-msq_x4) * xd[i];
+          r[i] = bs[i] + ttt[i];
+          p[i] = r[i];
+          return r[i] * r[i];
+        });
+        if (relrsqmin > 0) relrsq = relative_residue(r.data(), xd);
+        final_rsq = (T)rsq / source_norm;
+        final_rel = (T)relrsq;
+        iteration++;
+        if (iteration >= max_cg || nrestart >= max_restarts ||
+            ((rsqmin <= 0 || rsqmin > final_rsq) && (relrsqmin <= 0 || relrsqmin > final_rel)))
+          break;
+        if (iteration != 0) nrestart++;
+      }
+      oldrsq = rsq;
+      dslash(L, tmp.data(), p.data(), other);
+      dslash(L, ttt.data(), tmp.data(), parity);
+      /* calc_pkp, ks_d_congrad_fn.cpp:159-171 */
+      pkp = blocked_sum<T>(n, [&](long i) {
+        ttt[i] = ttt[i] + (-msq_x4) * p[i];
+        return p[i] * ttt[i];
+      });
+      iteration++;
+      const T a = (T)(-rsq / pkp);
+      /* calc_rsq2, ks_d_congrad_fn.cpp:131-147 */
+      rsq = blocked_sum<T>(n, [&](long i) {
+        xd[i] = xd[i] + a * p[i];
+        r[i] = r[i] + a * ttt[i];
+        return r[i] * r[i];
+      });
+      if (relrsqmin > 0) relrsq = relative_residue(r.data(), xd);
+      size_r = (T)rsq / source_norm;
+      size_relr = (T)relrsq;
+      final_rsq = size_r;
+      const T b = (T)rsq / oldrsq;
+      /* axpy, ks_d_congrad_fn.cpp:178-188 */
+#pragma omp parallel for
+      for (long i = 0; i < n; ++i) p[i] = r[i] + b * p[i];
+    }
+  }
+  total_iters += iteration;
+  o->final_rsq = final_rsq;
+  o->final_rel = final_rel;
+  o->size_r = size_r;
+  o->size_relr = size_relr;
+  o->final_iter = total_iters;
+  o->final_restart = nrestart;
+  o->final_flop = 1187.0 * total_iters * 2.0 * G.vh;
+  o->ret = iteration;
+}
+
+/* ks_multicg_offset.cpp:205-689 restated.  x[j] full raw arrays. */
+template <typename T>
+void invert_multi(const Links<T> *L, int parity_api, int niter, int max_restarts, double resid0,
+                  const T *mass, int num_offsets, T **x_full, const T *b_full, InvertOut *o) {
+  o->ret = 0;
+  if (num_offsets == 0) return;
+  const long n = 6 * G.vh;
+  std::vector<T> ttt(n), tmp(n), r(n);
+  std::vector<std::vector<T>> pm(num_offsets, std::vector<T>(n));
+  std::vector<T> shifts(num_offsets), zeta_i(num_offsets), zeta_im1(num_offsets),
+      zeta_ip1(num_offsets), beta_i(num_offsets), beta_im1(num_offsets), alpha(num_offsets);
+  std::vector<int> finished(num_offsets, 0);
+  const T rsqmin = (T)resid0;
+  const int max_cg = max_restarts * niter;
+  double offset_low = 1.0e+20;
+  int j_low = -1;
+  for (int j = 0; j < num_offsets; ++j) {
+    shifts[j] = (T)(mass[j] * mass[j] * 4.0);
+    if (shifts[j] < offset_low) { offset_low = shifts[j]; j_low = j; }
+  }
+  for (int j = 0; j < num_offsets; ++j)
+    if (j != j_low) shifts[j] -= shifts[j_low];
+  int iteration = 0, total_iters = 0;
+  double rsq = 0, oldrsq, pkp, source_norm = 0;
+  int todo[2], ntodo;
+  if (parity_api == P_EVENODD) { todo[0] = 0; todo[1] = 1; ntodo = 2; }
+  else { todo[0] = parity_api == P_EVEN ? 0 : 1; ntodo = 1; }
+  for (int pass = 0; pass < ntodo; ++pass) {
+    const int parity = todo[pass], other = 1 - parity;
+    const long voff = parity ? n : 0;
+    const T *bs = b_full + voff;
+    iteration = 0;
+    int num_now = num_offsets;
+    source_norm = blocked_sum<T>(n, [&](long i) { return bs[i] * bs[i]; });
+    for (int j = 0; j < num_offsets; ++j) {
+      std::memset(x_full[j] + voff, 0, sizeof(T) * n);
+      std::memcpy(pm[j].data(), bs, sizeof(T) * n);
+      finished[j] = 0;
+    }
+    std::memcpy(r.data(), bs, sizeof(T) * n);
+    rsq = source_norm;
+    if (source_norm == 0.0) continue;
+    const double rsqstop = rsqmin * source_norm;
+    for (int j = 0; j < num_offsets; ++j) {
+      zeta_im1[j] = zeta_i[j] = 1.0;
+      beta_im1[j] = -1.0;
+      alpha[j] = 0.0;
+    }
+    bool converged = false;
+    do {
+      oldrsq = rsq;
+      dslash(L, tmp.data(), pm[j_low].data(), other);
+      dslash(L, ttt.data(), tmp.data(), parity);
+      const T sh = (T)(-1.0 * offset_low);
+      T *pl = pm[j_low].data();
+      pkp = blocked_sum<T>(n, [&](long i) {
+        ttt[i] = ttt[i] + sh * pl[i];
+        return pl[i] * ttt[i];
+      });
+      iteration++;
+      total_iters++;
+      beta_i[j_low] = (T)(-rsq / pkp);
+      zeta_ip1[j_low] = 1.0;
+      for (int j = 0; j < num_now; ++j) {
+        if (j == j_low) continue;
+        zeta_ip1[j] = zeta_i[j] * zeta_im1[j] * beta_im1[j_low];
+        const T c1 = beta_i[j_low] * alpha[j_low] * (zeta_im1[j] - zeta_i[j]);
+        const T c2 = zeta_im1[j] * beta_im1[j_low] * (1.0 + shifts[j] * beta_i[j_low]);
+        if (c1 + c2 != (T)0.0) zeta_ip1[j] /= c1 + c2;
+        else { zeta_ip1[j] = 0.0; finished[j] = 1; }
+        if (zeta_i[j] != (T)0.0) beta_i[j] = beta_i[j_low] * zeta_ip1[j] / zeta_i[j];
+        else { zeta_ip1[j] = 0.0; beta_i[j] = 0.0; finished[j] = 1; }
+        if (finished[j] == 1 && j == num_now - 1 && j > j_low) num_now--;
+      }
+      const T bl = beta_i[j_low];
+      rsq = blocked_sum<T>(n, [&](long i) {
+        r[i] = r[i] + bl * ttt[i];
+        return r[i] * r[i];
+      });
+      if (rsq <= rsqstop) {
+        /* update_dest, ks_multicg_offset.cpp:142-159 */
+        for (int j = 0; j < num_now; ++j) {
+          T *xd = x_full[j] + voff;
+          const T *pj = pm[j].data();
+          const T bj = beta_i[j];
+#pragma omp parallel for
+          for (long i = 0; i < n; ++i) xd[i] = xd[i] + bj * pj[i];
+        }
+        converged = true;
+        break;
+      }
+      alpha[j_low] = (T)(rsq / oldrsq);
+      for (int j = 0; j < num_now; ++j) {
+        if (j == j_low) continue;
+        if (zeta_i[j] * beta_i[j_low] != (T)0.0)
+          alpha[j] = alpha[j_low] * zeta_ip1[j] * beta_i[j] / (zeta_i[j] * beta_i[j_low]);
+        else { alpha[j] = 0.0; finished[j] = 1; }
+      }
+      /* update_pm, ks_multicg_offset.cpp:161-184 */
+      for (int j = 0; j < num_now; ++j) {
+        T *xd = x_full[j] + voff;
+        T *pj = pm[j].data();
+        const T bj = beta_i[j], zj = zeta_ip1[j], aj = alpha[j];
+#pragma omp parallel for
+        for (long i = 0; i < n; ++i) {
+          xd[i] = xd[i] + bj * pj[i];
+          const T tt = r[i] * zj;
+          pj[i] = tt + aj * pj[i];
+        }
+      }
+      for (int j = 0; j < num_now; ++j) {
+        beta_im1[j] = beta_i[j];
+        zeta_im1[j] = zeta_i[j];
+        zeta_i[j] = zeta_ip1[j];
+      }
+    } while (iteration < max_cg);
+    (void)converged;
+  }
+  o->final_rsq = source_norm != 0 ? (T)rsq / source_norm : 0.0;
+  o->size_r = o->final_rsq;
+  o->final_iter = iteration;
+  o->final_restart = 0;
+  o->final_flop = (1205.0 + 15.0 * num_offsets) * total_iters * 2.0 * G.vh;
+  o->ret = iteration;
+}
+
+} // namespace
+
+extern "C" {
+
+/* Lattice set-up; the reference's minimum-size checks qphix_misc.cpp:494-497. */
+int oracle_setup(const int *latsize) {
+  if (latsize[0] % 4 || latsize[1] % 4 || latsize[2] % 4 || latsize[3] % 4) return 1;
+  G.nx = latsize[0]; G.ny = latsize[1]; G.nz = latsize[2]; G.nt = latsize[3];
+  G.nxh = G.nx / 2;
+  G.vh = (long)G.nxh * G.ny * G.nz * G.nt;
+  build_neighbours();
+  return 0;
+}
+
+/* Neighbour table of one parity, 16 entries per site (test access). */
+void oracle_neighbours(int cb, unsigned *out) {
+  std::memcpy(out, G.neigh[cb].data(), sizeof(unsigned) * 16 * G.vh);
+}
+
+int oracle_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void *oracle_links_create_d(const double *f, const double *l, const double *fb, const double *lb) {
+  return links_create<double>(f, l, fb, lb);
+}
+void *oracle_links_create_f(const float *f, const float *l, const float *fb, const float *lb) {
+  return links_create<float>(f, l, fb, lb);
+}
+void oracle_links_destroy_d(void *L) { delete (Links<double> *)L; }
+void oracle_links_destroy_f(void *L) { delete (Links<float> *)L; }
+
+/* cb = output parity (0 even, 1 odd); half-lattice vectors */
+void oracle_dslash_d(void *L, double *out, const double *in, int cb) {
+  dslash((Links<double> *)L, out, in, cb);
+}
+void oracle_dslash_f(void *L, float *out, const float *in, int cb) {
+  dslash((Links<float> *)L, out, in, cb);
+}
+
+double oracle_relative_residue_d(const double *r, const double *x) { return relative_residue(r, x); }
+
+int oracle_invert_d(void *L, int parity, int niter, int nrestart, double resid, double relresid,
+                    double mass, double *x, const double *b, double *out_d, int *out_i) {
+  InvertOut o{};
+  invert((Links<double> *)L, parity, niter, nrestart, resid, relresid, mass, x, b, &o);
+  out_d[0] = o.final_rsq; out_d[1] = o.final_rel; out_d[2] = o.size_r; out_d[3] = o.size_relr;
+  out_d[4] = o.final_flop;
+  out_i[0] = o.final_iter; out_i[1] = o.final_restart;
+  return o.ret;
+}
+int oracle_invert_f(void *L, int parity, int niter, int nrestart, double resid, double relresid,
+                    float mass, float *x, const float *b, double *out_d, int *out_i) {
+  InvertOut o{};
+  invert((Links<float> *)L, parity, niter, nrestart, resid, relresid, mass, x, b, &o);
+  out_d[0] = o.final_rsq; out_d[1] = o.final_rel; out_d[2] = o.size_r; out_d[3] = o.size_relr;
+  out_d[4] = o.final_flop;
+  out_i[0] = o.final_iter; out_i[1] = o.final_restart;
+  return o.ret;
+}
+
+int oracle_invert_multi_d(void *L, int parity, int niter, int nrestart, double resid0,
+                          const double *mass, int nmass, double **x, const double *b,
+                          double *out_d, int *out_i) {
+  InvertOut o{};
+  invert_multi((Links<double> *)L, parity, niter, nrestart, resid0, mass, nmass, x, b, &o);
+  out_d[0] = o.final_rsq; out_d[1] = o.final_flop;
+  out_i[0] = o.final_iter;
+  return o.ret;
+}
+int oracle_invert_multi_f(void *L, int parity, int niter, int nrestart, double resid0,
+                          const float *mass, int nmass, float **x, const float *b,
+                          double *out_d, int *out_i) {
+  InvertOut o{};
+  invert_multi((Links<float> *)L, parity, niter, nrestart, resid0, mass, nmass, x, b, &o);
+  out_d[0] = o.final_rsq; out_d[1] = o.final_flop;
+  out_i[0] = o.final_iter;
+  return o.ret;
+}
+
+} /* extern "C" */

@@ -1,0 +1,129 @@
+"""CG and multi-shift CG parity: HIP library (C ABI) vs the CPU oracle.
+
+Bar: the same true residual is reached, iteration counts within 5% (the
+library reduces in FP64 where the reference's F build reduces in float, so F
+counts may differ slightly), FP64 solutions within 1e-9 relative L2 of the
+oracle's, FP32 solutions within 1e-4 of the FP64 answer.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import lattice_session
+from helpers import full_from_half, half, random_links, rel_l2
+from pyqphix import api as A
+from pyqphix import fields as F
+
+pytestmark = pytest.mark.gpu
+NP = {"D": np.float64, "F": np.float32}
+
+# 8^4, fat = 1 + 0.1 Gaussian, long = Haar SU(3), m = 0.05 in
+# M = m^2 - D_eo D_oe, i.e. API mass m/2 (the API solves 4 mass^2 - D D);
+# CG to 1e-8 relative residual = resid 1e-16 (squared).
+CFG1 = dict(dims=(8, 8, 8, 8), kind="haar", seed=1001, mass=0.025, resid=1e-16)
+
+
+def setup(cfg, prec):
+    s = lattice_session(cfg["dims"])
+    lat = F.Lattice(cfg["dims"])
+    links = random_links(lat, cfg["kind"], cfg["seed"])
+    raw = [F.to_raw(a) for a in links]
+    L = A.Links(s, *[r.astype(NP[prec]) for r in raw], prec=prec)
+    orc = O.Oracle(lat.dims, *raw, dtype=NP[prec])
+    return s, lat, raw, L, orc
+
+
+def true_rsq(orc64, lat, x_full, b_full, mass, cb):
+    """|b - (4m^2 - D D) x|^2 / |b|^2 with the FP64 oracle operator."""
+    xh = half(x_full, lat, cb).astype(np.float64)
+    bh = half(b_full, lat, cb).astype(np.float64)
+    ddx = orc64.dslash(orc64.dslash(xh, 1 - cb), cb)
+    r = bh - (4 * mass * mass * xh - ddx)
+    return float(r @ r / (bh @ bh))
+
+
+@pytest.mark.parametrize("prec", ["D", "F"])
+@pytest.mark.parametrize("parity", [A.EVEN, A.ODD])
+def test_cg_config1_matches_oracle(prec, parity):
+    cfg = CFG1
+    s, lat, raw, L, orc = setup(cfg, prec)
+    cb = 0 if parity == A.EVEN else 1
+    b = F.to_raw(F.gaussian_source(lat, 77, parity=cb), NP[prec])
+    vb = A.Vector(s, b, parity, prec)
+    vx = A.Vector(s, None, parity, prec)
+    print(f"[case] cg prec={prec} parity={parity}", flush=True)
+    ret, info, res = A.invert(L, vx, vb, cfg["mass"], parity, cfg["resid"])
+    x = vx.extract()
+    xo, io = orc.invert(b, cfg["mass"], parity, cfg["resid"])
+    assert info.status == 0
+    assert abs(res.final_iter - io["final_iter"]) <= max(1, 0.05 * io["final_iter"])
+    assert ret == res.final_iter  # single parity: return value = total
+    orc64 = O.Oracle(lat.dims, *raw)
+    tr = true_rsq(orc64, lat, x, b, cfg["mass"], cb)
+    if prec == "D":
+        assert res.final_rsq < cfg["resid"] and tr < 1.5 * cfg["resid"]
+        assert rel_l2(half(x, lat, cb), half(xo, lat, cb)) < 1e-9
+        # same stopping residual up to CG's rounding-order sensitivity
+        assert io["final_rsq"] < cfg["resid"]
+        assert abs(np.log10(res.final_rsq / io["final_rsq"])) < 0.3
+    else:
+        # pure FP32 stagnates near eps_float * cond(M) (~1e-4 relative here),
+        # as the reference's F build does
+        assert tr < 1e-6
+        assert rel_l2(half(x, lat, cb), half(xo, lat, cb)) < 2e-3
+    assert info.final_flop == pytest.approx(1187.0 * res.final_iter * lat.V)
+    for o in (vb, vx, L):
+        o.free()
+
+
+def test_cg_restarts_evenodd_and_guess():
+    cfg = dict(CFG1, dims=(4, 4, 8, 8), seed=1003)
+    s, lat, raw, L, orc = setup(cfg, "D")
+    b = F.to_raw(F.gaussian_source(lat, 78))
+    vb = A.Vector(s, b, A.EVENODD)
+    vx = A.Vector(s, None, A.EVENODD)
+    # restarts every 40 iterations
+    ret, info, res = A.invert(L, vx, vb, 0.1, A.EVENODD, 1e-20, niter=40, nrestart=50)
+    xo, io = orc.invert(b, 0.1, O.EVENODD, 1e-20, niter=40, nrestart=50)
+    assert res.final_restart == io["final_restart"] and res.final_restart >= 1
+    assert abs(res.final_iter - io["final_iter"]) <= max(1, 0.05 * io["final_iter"])
+    assert ret == io["ret"]
+    assert rel_l2(vx.extract(), xo) < 1e-9
+    # the solution is used as the initial guess of the next call (C10)
+    ret2, info2, res2 = A.invert(L, vx, vb, 0.1, A.EVENODD, 1e-18, niter=40, nrestart=50)
+    assert res2.final_iter == 2  # one true-residual evaluation per parity
+    # zero source gives zero solution without iterations
+    vz = A.Vector(s, np.zeros(6 * lat.V), A.EVEN)
+    vxz = A.Vector(s, np.ones(6 * lat.V), A.EVEN)
+    ret3, info3, res3 = A.invert(L, vxz, vz, 0.1, A.EVEN, 1e-10)
+    assert ret3 == 0 and res3.final_iter == 0 and not np.any(vxz.extract())
+    for o in (vb, vx, vz, vxz, L):
+        o.free()
+
+
+@pytest.mark.parametrize("prec", ["D", "F"])
+def test_multishift_matches_oracle(prec):
+    cfg = dict(CFG1, seed=1004)
+    s, lat, raw, L, orc = setup(cfg, prec)
+    masses = [0.2, 0.025, 0.05, 0.1]
+    b = F.to_raw(F.gaussian_source(lat, 79, parity=0), NP[prec])
+    vb = A.Vector(s, b, A.EVEN, prec)
+    outs = [A.Vector(s, None, A.EVEN, prec) for _ in masses]
+    print(f"[case] multishift prec={prec}", flush=True)
+    resid = 1e-20 if prec == "D" else 1e-12
+    ret, info, res = A.invert_multi(L, outs, vb, masses, A.EVEN, resid)
+    xo, io = orc.invert_multi(b, np.array(masses, dtype=NP[prec]), O.EVEN, resid)
+    assert abs(ret - io["ret"]) <= max(1, 0.05 * io["ret"])
+    assert all(r.final_iter == ret for r in res)
+    orc64 = O.Oracle(lat.dims, *raw)
+    for j, m in enumerate(masses):
+        x = outs[j].extract()
+        if prec == "D":
+            assert rel_l2(half(x, lat, 0), half(xo[j], lat, 0)) < 1e-9
+            assert true_rsq(orc64, lat, x, b, m, 0) < 100 * resid
+        else:
+            assert rel_l2(half(x, lat, 0), half(xo[j], lat, 0)) < 2e-3
+            assert true_rsq(orc64, lat, x, b, m, 0) < 1e-5
+    assert info.final_flop == pytest.approx((1205.0 + 15.0 * len(masses)) * ret * lat.V)
+    for o in outs + [vb, L]:
+        o.free()
