@@ -67,6 +67,41 @@ def cpu_baseline(dims, raw, src, budget_s):
     return dt, reps, threads
 
 
+def cg_compare(A, s, L64, lat, src):
+    """FP64 CG vs mixed-precision CG (FP32 and FP16-stored sloppy spinors, FP64
+    reliable updates), each to a 1e-10 true relative residual (resid is the
+    squared residual, 1e-20), m = 0.02 in M = m^2 - D_eo D_oe (API mass m/2)."""
+    mass, resid = 0.01, 1e-20
+    b = src.copy()
+    b[6 * lat.vh:] = 0
+    vb = A.Vector(s, b, A.EVEN, "D")
+    out = {}
+    x64 = None
+    for name, bits in (("f64", None), ("mixed_f32", 32), ("mixed_f16", 16)):
+        vx = A.Vector(s, None, A.EVEN, "D")
+        t = time.perf_counter()
+        if bits is None:
+            ret, info, res = A.invert(L64, vx, vb, mass, A.EVEN, resid, niter=5000, nrestart=10)
+        else:
+            ret, info, res = A.invert_mixed(L64, vx, vb, mass, A.EVEN, resid, sloppy_bits=bits,
+                                            niter=5000, nrestart=10)
+        sec = time.perf_counter() - t
+        x = vx.extract()[:6 * lat.vh]
+        rec = {"seconds": round(sec, 4), "iters": res.final_iter, "final_rsq": res.final_rsq,
+               "status": info.status}
+        if bits is None:
+            x64 = x
+        else:
+            rec["reliable_updates"] = res.final_restart
+            rec["rel_err_vs_f64"] = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
+            rec["speedup_vs_f64"] = round(out["f64"]["seconds"] / sec, 3)
+        out[name] = rec
+        log(f"[bench] CG {name}: {sec:.3f}s {res.final_iter} iters rsq {res.final_rsq:.2e}")
+        vx.free()
+    vb.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,20 +171,7 @@ def main():
     del raw32
 
     if not args.no_cg:
-        # FP64 CG to a 1e-10 true relative residual (resid is squared: 1e-20), m = 0.02
-        mass = 0.01  # API mass m/2 for M = m^2 - D_eo D_oe with m = 0.02
-        b = src.copy()
-        b[6 * lat.vh:] = 0
-        vb = A.Vector(s, b, A.EVEN, "D")
-        vx = A.Vector(s, None, A.EVEN, "D")
-        t = time.perf_counter()
-        ret, info, res = A.invert(L64, vx, vb, mass, A.EVEN, 1e-20, niter=5000, nrestart=10)
-        cg_s = time.perf_counter() - t
-        extra["cg_f64"] = {"seconds": round(cg_s, 4), "iters": res.final_iter,
-                           "final_rsq": res.final_rsq, "restarts": res.final_restart}
-        log(f"[bench] FP64 CG {cg_s:.3f}s {res.final_iter} iters rsq {res.final_rsq:.2e}")
-        vb.free()
-        vx.free()
+        extra["cg"] = cg_compare(A, s, L64, lat, src)
 
     cpu = None
     if not args.no_cpu:

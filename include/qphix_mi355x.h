@@ -50,6 +50,23 @@ int QPHIX_F3_asqtad_links_recon(QPHIX_F3_FermionLinksAsqtad *L);
  * links in out[2..3]. */
 void QPHIX_D3_asqtad_links_backcheck(QPHIX_D3_FermionLinksAsqtad *L, double out[4]);
 
+/* ---- solvers ----------------------------------------------------------------
+ * Mixed-precision CG with reliable updates for the system of
+ * QPHIX_D3_asqtad_invert (ks_d_congrad_fn.cpp:221-619 solves it in one
+ * precision): every CG step runs with FP32 links and FP32 (sloppy_bits = 32)
+ * or FP16-stored, FP32-computed (sloppy_bits = 16) vectors; the solution, the
+ * true residual b - Mx (recomputed with the FP64 Dslash whenever the sloppy
+ * residual has dropped by 10x, or on apparent convergence) and all reductions
+ * are FP64.  Stops when the true squared relative residual is below
+ * res_arg->resid, or after inv_arg->max * inv_arg->nrestart iterations.
+ * out is the initial guess.  res_arg->final_restart returns the number of
+ * reliable updates; final_iter counts sloppy steps plus true-residual
+ * evaluations.  Returns the iterations of the last parity solved. */
+int QPHIX_D3_asqtad_invert_mixed(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad *asqtad,
+                                 QPHIX_invert_arg_t *inv_arg, QPHIX_resid_arg_t *res_arg,
+                                 double mass, QPHIX_D3_ColorVector *out, QPHIX_D3_ColorVector *in,
+                                 int sloppy_bits);
+
 /* ---- timing ----------------------------------------------------------------
  * Runs `reps` single-parity Dslash applications back to back on the library's
  * stream and returns the average kernel time in milliseconds measured with

@@ -151,28 +151,40 @@ static inline int nb(int vh) { return (vh + BLAS_BLOCK - 1) / BLAS_BLOCK; }
 ReduceCtl reduce_ctl(SolverState *st, int op) {
   ReduceCtl rc;
   rc.partials = ctx().partials;
-  rc.counter = ctx().counter;
   rc.state = st;
   rc.op = op;
-  rc.out = st ? nullptr : ctx().partials + 4 * (size_t)(ctx().max_blocks - 1);
+  rc.out = st ? nullptr : reduce_result_ptr();
   return rc;
 }
 
 double *reduce_result_ptr() { return ctx().partials + 4 * (size_t)(ctx().max_blocks - 1); }
 
+void reduce_finish(int nv, int nblocks, const ReduceCtl &rc, const int *gate) {
+  if (nv == 1)
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream, rc.partials,
+                       nblocks, rc, gate);
+  else
+    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream, rc.partials,
+                       nblocks, rc, gate);
+  HIP_CHECK(hipGetLastError());
+}
+
 template <typename T> void blas_norm2(const T *v, SolverState *st, int op) {
   const int vh = ctx().g.vh;
-  hipLaunchKernelGGL(k_norm2<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, v, vh,
-                     reduce_ctl(st, op));
+  const ReduceCtl rc = reduce_ctl(st, op);
+  hipLaunchKernelGGL(k_norm2<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, v, vh, rc);
   HIP_CHECK(hipGetLastError());
+  reduce_finish(1, nb(vh), rc, nullptr);
 }
 
 template <typename T>
 void blas_cg_update(T *x, const T *p, T *r, const T *ttt, SolverState *st, const int *gate) {
   const int vh = ctx().g.vh;
+  const ReduceCtl rc = reduce_ctl(st, SOP_CG_RSQ2);
   hipLaunchKernelGGL(k_cg_update<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, x, p, r,
-                     ttt, vh, reduce_ctl(st, SOP_CG_RSQ2), gate);
+                     ttt, vh, rc, gate);
   HIP_CHECK(hipGetLastError());
+  reduce_finish(2, nb(vh), rc, gate);
 }
 
 template <typename T> void blas_cg_axpy(const T *r, T *p, const SolverState *st, const int *gate) {
@@ -184,9 +196,11 @@ template <typename T> void blas_cg_axpy(const T *r, T *p, const SolverState *st,
 
 template <typename T> void blas_ms_update_r(T *r, const T *ttt, SolverState *st, const int *gate) {
   const int vh = ctx().g.vh;
+  const ReduceCtl rc = reduce_ctl(st, SOP_MS_RSQ2);
   hipLaunchKernelGGL(k_ms_update_r<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, r, ttt, vh,
-                     reduce_ctl(st, SOP_MS_RSQ2), gate);
+                     rc, gate);
   HIP_CHECK(hipGetLastError());
+  reduce_finish(1, nb(vh), rc, gate);
 }
 
 template <typename T>

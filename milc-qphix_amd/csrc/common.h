@@ -65,9 +65,8 @@ struct Context {
   bool sync_calls = true;    // every API call synchronises before returning
   bool abort_on_error = true;
   int n_cu = 0;
-  // reduction scratch (device): partial sums, block counter
+  // reduction scratch (device): per-block partial sums
   double *partials = nullptr;
-  unsigned *counter = nullptr;
   int max_blocks = 0;
   // pinned host mirror for small scalar reads
   void *pinned = nullptr;

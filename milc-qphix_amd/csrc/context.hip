@@ -118,9 +118,7 @@ static QPHIX_status_t init_common(QPHIX_layout_t *layout) {
   // Reduction scratch: enough partials for one block per 64 sites, 4 values each.
   c.max_blocks = (g.vh + 63) / 64 + 1024;
   c.partials = (double *)dev_alloc(sizeof(double) * 4 * (size_t)c.max_blocks);
-  c.counter = (unsigned *)dev_alloc(sizeof(unsigned) * 64);
-  if (!c.partials || !c.counter) return QPHIX_MEM_ERROR;
-  HIP_CHECK(hipMemset(c.counter, 0, sizeof(unsigned) * 64));
+  if (!c.partials) return QPHIX_MEM_ERROR;
   HIP_CHECK(hipHostMalloc(&c.pinned, 1 << 16, hipHostMallocDefault));
   c.initialised = true;
   return QPHIX_SUCCESS;
@@ -131,7 +129,6 @@ static QPHIX_status_t finalize_common() {
   if (!c.initialised) return QPHIX_SUCCESS;
   HIP_CHECK(hipStreamSynchronize(c.stream));
   dev_free(c.partials);
-  dev_free(c.counter);
   dev_free(c.ws);
   if (c.pinned) HIP_CHECK(hipHostFree(c.pinned));
   HIP_CHECK(hipStreamDestroy(c.stream));

@@ -64,6 +64,12 @@ template <> struct Spin<double> {
 #pragma unroll
     for (int c = 0; c < 3; ++c) q[c] = make_double2(v[c].re, v[c].im);
   }
+  static __device__ __forceinline__ void store_nt(double *p, int i, const cplx<double> *v) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    d2 *q = reinterpret_cast<d2 *>(p + 6 * (size_t)i);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) __builtin_nontemporal_store(d2{v[c].re, v[c].im}, q + c);
+  }
 };
 
 template <> struct Spin<float> {
@@ -81,6 +87,12 @@ template <> struct Spin<float> {
     float2 *q = reinterpret_cast<float2 *>(p + 6 * (size_t)i);
 #pragma unroll
     for (int c = 0; c < 3; ++c) q[c] = make_float2(v[c].re, v[c].im);
+  }
+  static __device__ __forceinline__ void store_nt(float *p, int i, const cplx<float> *v) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    f2 *q = reinterpret_cast<f2 *>(p + 6 * (size_t)i);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) __builtin_nontemporal_store(f2{v[c].re, v[c].im}, q + c);
   }
 };
 
@@ -112,30 +124,40 @@ template <> struct Spin<HalfSite> {
     raw.w = __float_as_uint(m);
     *reinterpret_cast<uint4 *>(p + i) = raw;
   }
+  static __device__ __forceinline__ void store_nt(HalfSite *p, int i, const cplx<float> *v) {
+    store(p, i, v);
+  }
 };
 
 // ---------------------------------------------------------------------------
 // Link pair loads from the chunk-major layout (see common.h).
 // NC complex numbers per pair; chunk = 16 bytes.
 // ---------------------------------------------------------------------------
-template <typename T, int NC> struct PairLoad;
+typedef double nd2 __attribute__((ext_vector_type(2)));
+typedef float nf4 __attribute__((ext_vector_type(4)));
 
-template <int NC> struct PairLoad<double, NC> {
+// NT = true: streaming (non-temporal) loads, so links read once per
+// application do not evict the neighbour spinors from L2 / Infinity Cache.
+template <typename T, int NC, bool NT = false> struct PairLoad;
+
+template <int NC, bool NT> struct PairLoad<double, NC, NT> {
   static __device__ __forceinline__ void load(const double2 *base, int vh, int i, cplx<double> *u) {
+    const nd2 *b = reinterpret_cast<const nd2 *>(base);
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      const double2 d = base[(size_t)k * vh + i];
+      const nd2 d = NT ? __builtin_nontemporal_load(b + (size_t)k * vh + i) : b[(size_t)k * vh + i];
       u[k].re = d.x;
       u[k].im = d.y;
     }
   }
 };
 
-template <int NC> struct PairLoad<float, NC> {
+template <int NC, bool NT> struct PairLoad<float, NC, NT> {
   static __device__ __forceinline__ void load(const float4 *base, int vh, int i, cplx<float> *u) {
+    const nf4 *b = reinterpret_cast<const nf4 *>(base);
 #pragma unroll
     for (int k = 0; k < NC / 2; ++k) {
-      const float4 d = base[(size_t)k * vh + i];
+      const nf4 d = NT ? __builtin_nontemporal_load(b + (size_t)k * vh + i) : b[(size_t)k * vh + i];
       u[2 * k].re = d.x;
       u[2 * k].im = d.y;
       u[2 * k + 1].re = d.z;

@@ -16,15 +16,21 @@
 //   EPI_RESTART r = b + D in + s*x, p = r, reduce |r|^2 and the relative residue
 //                                                   (calc_rsq, ks_d_congrad_fn.cpp:82-99;
 //                                                    relative_residue, ks_blas_utils_c.cpp:110-149)
-#include "reduce.h"
+#include "blas.h"
 #include "dslash_kernel.h"
 
 namespace qphix {
 
+// Kernel variant used by every launch (see dslash_kernel.h).  Measured on
+// MI355X at 32^4 (Naik 12): XCD slabs + streaming link loads take FP64 from
+// 230 to 207 us and FP32 from 104 to 93 us; streaming output stores add ~2%
+// but evict the result the solvers read next, so they are off.
+constexpr int DSLASH_OPT = OPT_XCD | OPT_NTL;
+
 template <typename TL, int NL, typename SI, typename SO, int EPI>
 static void launch_one(const DslashKArgs<TL, SI, SO> &a, int nblocks) {
-  hipLaunchKernelGGL((k_dslash<TL, NL, SI, SO, EPI>), dim3(nblocks), dim3(DSLASH_BLOCK), 0,
-                     ctx().stream, a);
+  hipLaunchKernelGGL((k_dslash<TL, NL, SI, SO, EPI, DSLASH_OPT>), dim3(nblocks),
+                     dim3(DSLASH_BLOCK), 0, ctx().stream, a);
 }
 
 template <typename TL, typename SI, typename SO>
@@ -40,6 +46,8 @@ static void dispatch(const DslashKArgs<TL, SI, SO> &a, int recon, int epi) {
     else launch_one<TL, 18, SI, SO, EPI_RESTART>(a, nb);
   }
   HIP_CHECK(hipGetLastError());
+  if (epi == EPI_PKP) reduce_finish(1, nb, a.rc, a.gate);
+  else if (epi == EPI_RESTART) reduce_finish(2, nb, a.rc, a.gate);
 }
 
 static Geo make_geo() {
@@ -63,11 +71,7 @@ static DslashKArgs<TL, SI, SO> make_args(const LinksT<TL> &L, int cb, SO *out, c
   a.b = reinterpret_cast<const SI *>(ex.b);
   a.r = reinterpret_cast<SO *>(ex.r);
   a.p = reinterpret_cast<SO *>(ex.p);
-  a.rc.partials = ctx().partials;
-  a.rc.counter = ctx().counter;
-  a.rc.state = reinterpret_cast<SolverState *>(ex.state);
-  a.rc.op = ex.scalar_op;
-  a.rc.out = ex.state ? nullptr : ctx().partials + 4 * (size_t)(ctx().max_blocks - 1);
+  a.rc = reduce_ctl(reinterpret_cast<SolverState *>(ex.state), ex.scalar_op);
   a.gate = ex.gate;
   return a;
 }

@@ -22,15 +22,30 @@ template <typename TL, typename SI, typename SO> struct DslashKArgs {
   const int *gate;
 };
 
+// Kernel variants (compile-time bit mask):
+//   OPT_XCD  blocks are renumbered so that each of the 8 XCDs (blocks are
+//            dispatched round-robin over XCDs) sweeps one contiguous slab of
+//            sites, keeping its neighbour gathers inside its own L2
+//   OPT_NTL  links are loaded non-temporally (streamed, not cached)
+//   OPT_NTS  the output is stored non-temporally
+enum { OPT_XCD = 1, OPT_NTL = 2, OPT_NTS = 4 };
+constexpr int N_XCD = 8;
+
+__device__ __forceinline__ int block_slot(int opt) {
+  const int nb = gridDim.x, b = blockIdx.x;
+  if ((opt & OPT_XCD) && nb % N_XCD == 0) return (b % N_XCD) * (nb / N_XCD) + b / N_XCD;
+  return b;
+}
+
 // Applies one direction's link pair (forward link on in(fwd), back link on
 // in(bwd)).  NC = 18: both links complete; NC = 12: rows 0-1 of each, row 2
 // rebuilt as sign * conj(r0 x r1).
-template <typename TL, int NC, typename SI>
+template <typename TL, int NC, typename SI, bool NT>
 __device__ __forceinline__ void apply_pair(const typename vec16<TL>::type *base, int vh, int i,
                                            const SI *in, int nf, int nb, unsigned sg_f,
                                            unsigned sg_b, cplx<TL> *acc) {
   cplx<TL> u[NC];
-  PairLoad<TL, NC>::load(base, vh, i, u);
+  PairLoad<TL, NC, NT>::load(base, vh, i, u);
   cplx<TL> vf[3], vb[3];
   Spin<SI>::load(in, nf, vf);
   Spin<SI>::load(in, nb, vb);
@@ -50,13 +65,14 @@ __device__ __forceinline__ void apply_pair(const typename vec16<TL>::type *base,
   }
 }
 
-template <typename TL, int NL, typename SI, typename SO, int EPI>
+template <typename TL, int NL, typename SI, typename SO, int EPI, int OPT>
 __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash(const DslashKArgs<TL, SI, SO> a) {
   if (a.gate && *a.gate == 0) return;  // uniform over the grid
   constexpr int CPV = vec16<TL>::ncplx;
   constexpr int NCH_F = 18 / CPV, NCH_L = NL / CPV;
+  constexpr bool NTL = (OPT & OPT_NTL) != 0;
   const int vh = a.g.vh;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = block_slot(OPT) * blockDim.x + threadIdx.x;
   const bool valid = i < vh;
   cplx<TL> acc[3];
 #pragma unroll
@@ -68,14 +84,17 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash(const DslashKArgs<TL, S
     if constexpr (NL == 12) sg = a.lsign[i];
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) {
-      apply_pair<TL, 18, SI>(a.fat + (size_t)mu * NCH_F * vh, vh, i, a.in, n.f1[mu], n.b1[mu], 0,
-                             0, acc);
-      apply_pair<TL, NL, SI>(a.lng + (size_t)mu * NCH_L * vh, vh, i, a.in, n.f3[mu], n.b3[mu],
-                             (sg >> (2 * mu + 1)) & 1u, (sg >> (2 * mu)) & 1u, acc);
+      apply_pair<TL, 18, SI, NTL>(a.fat + (size_t)mu * NCH_F * vh, vh, i, a.in, n.f1[mu],
+                                  n.b1[mu], 0, 0, acc);
+      apply_pair<TL, NL, SI, NTL>(a.lng + (size_t)mu * NCH_L * vh, vh, i, a.in, n.f3[mu],
+                                  n.b3[mu], (sg >> (2 * mu + 1)) & 1u, (sg >> (2 * mu)) & 1u, acc);
     }
   }
   if constexpr (EPI == EPI_PLAIN) {
-    if (valid) Spin<SO>::store(a.out, i, acc);
+    if (valid) {
+      if constexpr ((OPT & OPT_NTS) != 0) Spin<SO>::store_nt(a.out, i, acc);
+      else Spin<SO>::store(a.out, i, acc);
+    }
   } else if constexpr (EPI == EPI_PKP) {
     double v[1] = {0.0};
     if (valid) {

@@ -1,0 +1,324 @@
+// mixed.hip -- mixed-precision CG with reliable updates.
+//
+// No counterpart in the reference (its solvers run in one precision per
+// translation unit, qcd_data_types.h:4-20).  Solves the same system as
+// QPHIX_D3_asqtad_invert, (4 m^2 - D D) x = b to a squared relative true
+// residual `resid`, with every CG step in FP32 (or with FP16-stored spinors and
+// FP32 arithmetic) on FP32 links, and FP64 kept for the solution, the true
+// residual recomputed with the FP64 Dslash at each reliable update, and all
+// reductions.  Each reliable update and each sloppy step counts as one
+// iteration (the reference's counting, C9).
+#include "blas.h"
+#include "handles.h"
+#include "qphix_mi355x.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+
+namespace qphix {
+
+// x_s += alpha p ; r_s += alpha Ap ; reduce |r_s|^2
+template <typename P>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_mx_step(float *xs, const P *p, float *rs, const float *Ap, int vh, ReduceCtl rc,
+              const int *gate) {
+  if (*gate == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[1] = {0.0};
+  if (i < vh) {
+    const float al = (float)rc.state->mx_alpha;
+    cplx<float> xv[3], pv[3], rv[3], av[3];
+    Spin<float>::load(xs, i, xv);
+    Spin<P>::load(p, i, pv);
+    Spin<float>::load(rs, i, rv);
+    Spin<float>::load(Ap, i, av);
+    for (int c = 0; c < 3; ++c) {
+      xv[c].re += al * pv[c].re;
+      xv[c].im += al * pv[c].im;
+      rv[c].re += al * av[c].re;
+      rv[c].im += al * av[c].im;
+      s[0] += (double)rv[c].re * rv[c].re + (double)rv[c].im * rv[c].im;
+    }
+    Spin<float>::store(xs, i, xv);
+    Spin<float>::store(rs, i, rv);
+  }
+  grid_reduce<1>(s, rc);
+}
+
+// p = r_s + beta p
+template <typename P>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_mx_axpy(const float *rs, P *p, int vh, const SolverState *st, const int *gate) {
+  if (*gate == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  const float be = (float)st->mx_beta;
+  cplx<float> rv[3], pv[3];
+  Spin<float>::load(rs, i, rv);
+  Spin<P>::load(p, i, pv);
+  for (int c = 0; c < 3; ++c) {
+    pv[c].re = rv[c].re + be * pv[c].re;
+    pv[c].im = rv[c].im + be * pv[c].im;
+  }
+  Spin<P>::store(p, i, pv);
+}
+
+// x += x_s ; x_s = 0
+__global__ void __launch_bounds__(BLAS_BLOCK) k_mx_accum(double *x, float *xs, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] += (double)xs[i];
+  xs[i] = 0.f;
+}
+
+// r_s = (float) r ; p = r_s ; x_s = 0
+template <typename P>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_mx_init(float *rs, const double *r, P *p, float *xs, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  cplx<float> v[3];
+  for (int c = 0; c < 3; ++c) {
+    v[c].re = (float)r[6 * (size_t)i + 2 * c];
+    v[c].im = (float)r[6 * (size_t)i + 2 * c + 1];
+  }
+  Spin<float>::store(rs, i, v);
+  Spin<P>::store(p, i, v);
+  const cplx<float> z[3] = {{0, 0}, {0, 0}, {0, 0}};
+  Spin<float>::store(xs, i, z);
+}
+
+// r_s = (float) r ; reduce r_s.p
+template <typename P>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_mx_rp(float *rs, const double *r, const P *p, int vh, ReduceCtl rc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[1] = {0.0};
+  if (i < vh) {
+    cplx<float> v[3], pv[3];
+    Spin<P>::load(p, i, pv);
+    for (int c = 0; c < 3; ++c) {
+      v[c].re = (float)r[6 * (size_t)i + 2 * c];
+      v[c].im = (float)r[6 * (size_t)i + 2 * c + 1];
+      s[0] += (double)v[c].re * pv[c].re + (double)v[c].im * pv[c].im;
+    }
+    Spin<float>::store(rs, i, v);
+  }
+  grid_reduce<1>(s, rc);
+}
+
+// p = r_s + beta (p - gamma r_s)
+template <typename P>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_mx_newp(const float *rs, P *p, int vh, const SolverState *st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  const float be = (float)st->mx_beta, ga = (float)st->mx_gamma;
+  cplx<float> rv[3], pv[3];
+  Spin<float>::load(rs, i, rv);
+  Spin<P>::load(p, i, pv);
+  for (int c = 0; c < 3; ++c) {
+    pv[c].re = rv[c].re + be * (pv[c].re - ga * rv[c].re);
+    pv[c].im = rv[c].im + be * (pv[c].im - ga * rv[c].im);
+  }
+  Spin<P>::store(p, i, pv);
+}
+
+__global__ void k_mx_start(SolverState *s) {
+  s->mx_rsq = s->rsq;
+  s->mx_rsq_old = s->rsq;
+  s->mx_maxr = sqrt(s->rsq);
+  s->active = 1;
+}
+
+static double wall_now() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+static const size_t HEAD = offsetof(SolverState, shifts);
+
+static void read_state(const SolverState *d, SolverState &h) {
+  HIP_CHECK(hipMemcpyAsync(ctx().pinned, d, sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
+  HIP_CHECK(hipStreamSynchronize(ctx().stream));
+  std::memcpy(&h, ctx().pinned, sizeof(SolverState));
+}
+
+static void write_state(SolverState *d, const SolverState &h) {
+  std::memcpy(ctx().pinned, &h, sizeof(SolverState));
+  HIP_CHECK(hipMemcpyAsync(d, ctx().pinned, sizeof(SolverState), hipMemcpyHostToDevice, ctx().stream));
+  HIP_CHECK(hipStreamSynchronize(ctx().stream));
+}
+
+// Sloppy operator application Ap = -(4m^2 - D D) p, reducing p.Ap.
+static void sloppy_apply(const LinksT<float> &Lf, int parity, float *p, float *tmp, float *Ap,
+                         double s, SolverState *st, const int *gate) {
+  DslashExtra e1;
+  e1.gate = gate;
+  launch_dslash<float>(Lf, 1 - parity, tmp, p, EPI_PLAIN, e1);
+  DslashExtra e2;
+  e2.gate = gate;
+  e2.s = s;
+  e2.aux = p;
+  e2.state = st;
+  e2.scalar_op = SOP_MX_PAP;
+  launch_dslash<float>(Lf, parity, Ap, tmp, EPI_PKP, e2);
+}
+
+static void sloppy_apply(const LinksT<float> &Lf, int parity, HalfSite *p, HalfSite *tmp,
+                         float *Ap, double s, SolverState *st, const int *gate) {
+  DslashExtra e1;
+  e1.gate = gate;
+  launch_dslash_half(Lf, 1 - parity, tmp, nullptr, p, EPI_PLAIN, e1);
+  DslashExtra e2;
+  e2.gate = gate;
+  e2.s = s;
+  e2.aux = p;
+  e2.state = st;
+  e2.scalar_op = SOP_MX_PAP;
+  launch_dslash_half(Lf, parity, nullptr, Ap, tmp, EPI_PKP, e2);
+}
+
+template <typename P>
+static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
+                        QPHIX_resid_arg_t *res, double mass, VectorT<double> *out,
+                        VectorT<double> *in) {
+  const double t0 = wall_now();
+  const int vh = ctx().g.vh;
+  const size_t nv = 6 * (size_t)vh;
+  if (!L->sloppy) {
+    L->sloppy = new LinksT<float>;
+    links_to_float(*L, *L->sloppy);
+  }
+  const LinksT<float> &Lf = *L->sloppy;
+  const size_t psz = sizeof(P) == sizeof(HalfSite) ? sizeof(HalfSite) * (size_t)vh : nv * sizeof(float);
+  char *ws = (char *)ws_get(2 * nv * sizeof(double) + 2 * psz + 3 * nv * sizeof(float) +
+                            sizeof(SolverState) + 1024);
+  double *r64 = (double *)ws, *tmp64 = r64 + nv;
+  P *p = (P *)(tmp64 + nv);
+  P *tmps = (P *)((char *)p + psz);
+  float *rs = (float *)((char *)tmps + psz), *xs = rs + nv, *Ap = xs + nv;
+  SolverState *dst = (SolverState *)(((uintptr_t)(Ap + nv) + 255) & ~(uintptr_t)255);
+  int *gate = &dst->active;
+  const double msq_x4 = 4.0 * mass * mass;
+  const int nb = (vh + BLAS_BLOCK - 1) / BLAS_BLOCK;
+  const int nbn = (int)((nv + BLAS_BLOCK - 1) / BLAS_BLOCK);
+  const char *de = std::getenv("QPHIX_MIXED_DELTA");
+  const int batch = 16;
+
+  SolverState hs{};
+  hs.niter = inv->max;
+  hs.max_cg = inv->max * inv->nrestart;
+  hs.mx_delta = de ? std::atof(de) : 0.1;
+  int todo[2], ntodo;
+  if (inv->parity == QPHIX_EVENODD) { todo[0] = 0; todo[1] = 1; ntodo = 2; }
+  else { todo[0] = inv->parity == QPHIX_EVEN ? 0 : 1; ntodo = 1; }
+  int total = 0, updates = 0, iteration = 0;
+  info->status = QPHIX_SUCCESS;
+  for (int pass = 0; pass < ntodo; ++pass) {
+    const int parity = todo[pass], other = 1 - parity;
+    double *x = out->half[parity];
+    const double *b = in->half[parity];
+    hs.iteration = 0;
+    hs.active = 0;
+    write_state(dst, hs);
+    blas_norm2<double>(b, dst, SOP_SRC_NORM);
+    read_state(dst, hs);
+    if (hs.src_norm == 0.0) {
+      HIP_CHECK(hipMemsetAsync(x, 0, nv * sizeof(double), ctx().stream));
+      hs.final_rsq = 0;
+      iteration = 0;
+      continue;
+    }
+    hs.rsqstop = res->resid * hs.src_norm;
+    write_state(dst, hs);
+    bool first = true;
+    while (true) {
+      // reliable update: x += x_s, r = b - M x in FP64
+      if (!first) hipLaunchKernelGGL(k_mx_accum, dim3(nbn), dim3(BLAS_BLOCK), 0, ctx().stream, x, xs, (int)nv);
+      DslashExtra e1;
+      launch_dslash<double>(*L, other, tmp64, x, EPI_PLAIN, e1);
+      DslashExtra e2;
+      e2.s = -msq_x4;
+      e2.aux = x;
+      e2.b = b;
+      e2.r = r64;
+      e2.p = r64;
+      e2.state = dst;
+      e2.scalar_op = SOP_MX_TRUE;
+      launch_dslash<double>(*L, parity, r64, tmp64, EPI_RESTART, e2);
+      read_state(dst, hs);
+      if (hs.final_rsq < res->resid || hs.iteration >= hs.max_cg || !std::isfinite(hs.rsq)) break;
+      if (first) {
+        hipLaunchKernelGGL(k_mx_init<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, r64, p, xs, vh);
+        hipLaunchKernelGGL(k_mx_start, dim3(1), dim3(1), 0, ctx().stream, dst);
+        first = false;
+      } else {
+        const ReduceCtl rc = reduce_ctl(dst, SOP_MX_RP);
+        hipLaunchKernelGGL(k_mx_rp<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, r64, p, vh, rc);
+        reduce_finish(1, nb, rc, nullptr);
+        hipLaunchKernelGGL(k_mx_newp<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, vh, dst);
+        ++updates;
+      }
+      // sloppy CG steps until the device flags the next reliable update
+      while (true) {
+        for (int k = 0; k < batch; ++k) {
+          sloppy_apply(Lf, parity, p, tmps, Ap, -msq_x4, dst, gate);
+          const ReduceCtl rc = reduce_ctl(dst, SOP_MX_RSQ);
+          hipLaunchKernelGGL(k_mx_step<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs,
+                             Ap, vh, rc, gate);
+          reduce_finish(1, nb, rc, gate);
+          hipLaunchKernelGGL(k_mx_axpy<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, vh,
+                             dst, gate);
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(ctx().pinned, dst, HEAD, hipMemcpyDeviceToHost, ctx().stream));
+        HIP_CHECK(hipStreamSynchronize(ctx().stream));
+        std::memcpy(&hs, ctx().pinned, HEAD);
+        if (hs.active == 0) break;
+      }
+    }
+    iteration = hs.iteration;
+    total += iteration;
+    if (!std::isfinite(hs.rsq)) info->status = QPHIX_FAIL;
+  }
+  HIP_CHECK(hipStreamSynchronize(ctx().stream));
+  res->final_rsq = hs.final_rsq;
+  res->final_rel = 0;
+  res->size_r = hs.final_rsq;
+  res->size_relr = 0;
+  res->final_iter = total;
+  res->final_restart = updates;
+  info->final_sec = wall_now() - t0;
+  info->final_flop = 1187.0 * total * 2.0 * vh;
+  return iteration;
+}
+
+} // namespace qphix
+
+using namespace qphix;
+
+extern "C" {
+
+int QPHIX_D3_asqtad_invert_mixed(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad *L,
+                                 QPHIX_invert_arg_t *inv, QPHIX_resid_arg_t *res, double mass,
+                                 QPHIX_D3_ColorVector *out, QPHIX_D3_ColorVector *in,
+                                 int sloppy_bits) {
+  QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called: asqtad_invert_mixed", 0);
+  QPHIX_REQUIRE(info && L && inv && res && out && in, "asqtad_invert_mixed: null argument", 0);
+  info->status = QPHIX_FAIL;
+  QPHIX_REQUIRE(sloppy_bits == 32 || sloppy_bits == 16, "asqtad_invert_mixed: sloppy precision must be 32 or 16", 0);
+  QPHIX_REQUIRE(in->parity == out->parity && in->parity == (int)inv->parity,
+                "asqtad_invert_mixed: in, out and inv_arg parities must agree", 0);
+  QPHIX_REQUIRE(inv->max > 0 && inv->nrestart > 0, "asqtad_invert_mixed: bad iteration limits", 0);
+  QPHIX_REQUIRE(L->fat[0] && L->fat[1], "asqtad_invert_mixed: both link parities are required", 0);
+  for (int cb = 0; cb < 2; ++cb)
+    if (inv->parity & (cb ? QPHIX_ODD : QPHIX_EVEN))
+      QPHIX_REQUIRE(out->half[cb] && in->half[cb], "asqtad_invert_mixed: vector lacks a solved parity", 0);
+  if (sloppy_bits == 32) return invert_mixed<float>(info, L, inv, res, mass, out, in);
+  return invert_mixed<HalfSite>(info, L, inv, res, mass, out, in);
+}
+
+} // extern "C"

@@ -112,6 +112,18 @@ def invert(L, out, inp, mass, parity, resid, relresid=0.0, niter=1000, nrestart=
     return ret, info, res
 
 
+def invert_mixed(L, out, inp, mass, parity, resid, sloppy_bits=32, niter=1000, nrestart=5):
+    """QPHIX_D3_asqtad_invert_mixed: FP32 or FP16-stored sloppy CG with FP64
+    reliable updates (D3 links and vectors)."""
+    assert L.prec == "D"
+    info = Info()
+    inv = InvertArg(parity, niter, 0, nrestart, parity)
+    res = ResidArg(resid, 0, 0, 0, 0, 0, 0, 0)
+    ret = L.s.lib.QPHIX_D3_asqtad_invert_mixed(C.byref(info), L.h, C.byref(inv), C.byref(res),
+                                               mass, out.h, inp.h, sloppy_bits)
+    return ret, info, res
+
+
 def invert_multi(L, outs, inp, masses, parity, resid, niter=1000, nrestart=5):
     info = Info()
     inv = InvertArg(parity, niter, 0, nrestart, parity)

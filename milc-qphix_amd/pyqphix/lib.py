@@ -122,13 +122,9 @@ class Lib:
         _sig(lib, "QPHIX_MI355X_set_long_recon", None, [C.c_int])
         _sig(lib, "QPHIX_D3_asqtad_links_backcheck", None, [_dp, C.POINTER(C.c_double)])
         _sig(lib, "QPHIX_MI355X_copy_time", C.c_double, [C.c_size_t, C.c_int])
-        self._opt = {}
-        for name, res, args in (
-            ("QPHIX_D3_asqtad_invert_mixed", C.c_int,
-             [C.POINTER(Info), _dp, C.POINTER(InvertArg), C.POINTER(ResidArg), C.c_double, _dp, _dp, C.c_int]),
-        ):
-            if hasattr(lib, name):
-                self._opt[name] = _sig(lib, name, res, args)
+        _sig(lib, "QPHIX_D3_asqtad_invert_mixed", C.c_int,
+             [C.POINTER(Info), _dp, C.POINTER(InvertArg), C.POINTER(ResidArg), C.c_double, _dp,
+              _dp, C.c_int])
 
     def __getattr__(self, name):
         return getattr(self.lib, name)
