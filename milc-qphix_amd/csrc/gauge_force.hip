@@ -1,15 +1,232 @@
-// gauge_force.hip -- Symanzik one-loop gauge force (placeholder until the kernel lands).
+// gauge_force.hip -- Symanzik one-loop gauge force on the momentum.
+//
+// Replaces QPHIX_*3_symanzik_1loop_gauge_force (gauge_force_imp.cpp:136-199)
+// and its two-phase scatter (update_h_u_gforce_imp, :221-369, whose codegen
+// bodies gforce_imp_body / update_mg_body are absent).  Semantics:
+//   S(U) = - sum over geometric loops P of c_P Re Tr U_P,
+//   mom_mu(x) <- mom_mu(x) - eps * TA(W_mu(x)),
+//   W_mu(x) = U_mu(x) * sum_{loops P through (x,mu)} c_P * R_P,
+// where R_P is the rest of P after its first step +mu from x and
+// TA(M) = (M - M^dag)/2 - Tr(M - M^dag)/6.  c = coeffs->{plaquette, rectangle,
+// parallelogram}; adjoint_plaquette is ignored (as in the reference).  The
+// gauge field is not updated (the reference's update_mg_body may also touch U;
+// that cannot be determined without its absent body).
+//
+// MI355X form: gather, not scatter -- one work-item per link computes the 48
+// staples through its link (6 plaquette, 18 rectangle, 24 parallelogram) by
+// walking link paths from L2-resident neighbours, in FP64, and writes only its
+// own momentum.  No atomics, no second pass.
+#include "blas.h"
 #include "handles.h"
 
+#include <chrono>
+
+namespace qphix {
+
+struct GFGeo {
+  int L[4];
+  int nxh, vh;
+};
+
+template <typename T> struct GFArgs {
+  const T *u[2];
+  T *h[2];
+  GFGeo g;
+  double c[3];
+  double eps;
+};
+
+struct DM3 {
+  double re[9], im[9];
+};
+
+__device__ __forceinline__ void dm_mul(const DM3 &a, const DM3 &b, DM3 &r) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double sr = 0, si = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        sr = fma(a.re[3 * i + k], b.re[3 * k + j], sr);
+        sr = fma(-a.im[3 * i + k], b.im[3 * k + j], sr);
+        si = fma(a.re[3 * i + k], b.im[3 * k + j], si);
+        si = fma(a.im[3 * i + k], b.re[3 * k + j], si);
+      }
+      r.re[3 * i + j] = sr;
+      r.im[3 * i + j] = si;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_link(const GFArgs<T> &a, const int *y, int d, bool dag, DM3 &m) {
+  int c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int v = y[k];
+    v = v < 0 ? v + a.g.L[k] : (v >= a.g.L[k] ? v - a.g.L[k] : v);
+    c[k] = v;
+  }
+  const int p = (c[0] + c[1] + c[2] + c[3]) & 1;
+  const int i = (c[0] >> 1) + a.g.nxh * (c[1] + a.g.L[1] * (c[2] + a.g.L[2] * c[3]));
+  const T *s = a.u[p] + 72 * (size_t)i + 18 * d;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const int src = dag ? 3 * cc + r : 3 * r + cc;
+      m.re[3 * r + cc] = (double)s[2 * src];
+      m.im[3 * r + cc] = dag ? -(double)s[2 * src + 1] : (double)s[2 * src + 1];
+    }
+}
+
+// Product of the links along `n` signed steps (+(d+1) forward, -(d+1)
+// backward) from vertex y, accumulated into acc with weight c.
+template <typename T>
+__device__ void walk_acc(const GFArgs<T> &a, const int *y0, const int *steps, int n, double c,
+                         DM3 &acc) {
+  int y[4] = {y0[0], y0[1], y0[2], y0[3]};
+  DM3 r, m, t;
+  for (int k = 0; k < n; ++k) {
+    const int s = steps[k], d = (s > 0 ? s : -s) - 1;
+    if (s > 0) {
+      load_link(a, y, d, false, m);
+      y[d] += 1;
+    } else {
+      y[d] -= 1;
+      load_link(a, y, d, true, m);
+    }
+    if (k == 0) r = m;
+    else {
+      dm_mul(r, m, t);
+      r = t;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    acc.re[e] = fma(c, r.re[e], acc.re[e]);
+    acc.im[e] = fma(c, r.im[e], acc.im[e]);
+  }
+}
+
+template <typename T> __global__ void __launch_bounds__(256) k_gauge_force(const GFArgs<T> a) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nlink = 8 * a.g.vh;
+  if (l >= nlink) return;
+  const int cb = l / (4 * a.g.vh);
+  const int rem = l - cb * 4 * a.g.vh;
+  const int i = rem >> 2, mu = rem & 3;
+  int x[4];
+  {
+    const int xh = i % a.g.nxh;
+    int r = i / a.g.nxh;
+    x[1] = r % a.g.L[1];
+    r /= a.g.L[1];
+    x[2] = r % a.g.L[2];
+    x[3] = r / a.g.L[2];
+    x[0] = 2 * xh + ((x[1] + x[2] + x[3] + cb) & 1);
+  }
+  int y[4] = {x[0], x[1], x[2], x[3]};
+  y[mu] += 1;  // every rest path starts at x + mu
+  DM3 S;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) S.re[e] = S.im[e] = 0;
+  const int m = mu + 1;
+  for (int nu = 0; nu < 4; ++nu) {
+    if (nu == mu) continue;
+    for (int sg = -1; sg <= 1; sg += 2) {
+      const int n = sg * (nu + 1);
+      if (a.c[0] != 0) {
+        const int p[3] = {n, -m, -n};
+        walk_acc(a, y, p, 3, a.c[0], S);
+      }
+      if (a.c[1] != 0) {
+        const int ra[5] = {m, n, -m, -m, -n};
+        const int rb[5] = {n, -m, -m, -n, m};
+        const int rc[5] = {n, n, -m, -n, -n};
+        walk_acc(a, y, ra, 5, a.c[1], S);
+        walk_acc(a, y, rb, 5, a.c[1], S);
+        walk_acc(a, y, rc, 5, a.c[1], S);
+      }
+    }
+  }
+  if (a.c[2] != 0) {
+    for (int da = 0; da < 4; ++da) {
+      if (da == mu) continue;
+      for (int db = 0; db < 4; ++db) {
+        if (db == mu || db == da) continue;
+        for (int sa = -1; sa <= 1; sa += 2)
+          for (int sb = -1; sb <= 1; sb += 2) {
+            const int A = sa * (da + 1), B = sb * (db + 1);
+            const int p[5] = {A, B, -m, -A, -B};
+            walk_acc(a, y, p, 5, a.c[2], S);
+          }
+      }
+    }
+  }
+  DM3 U, W;
+  load_link(a, x, mu, false, U);
+  dm_mul(U, S, W);
+  // H <- H + i eps TA(W)  (momentum stored as H = -iA, 8 reals)
+  const double imtr = (W.im[0] + W.im[4] + W.im[8]) / 3.0;
+  T *h = a.h[cb] + 32 * (size_t)i + 8 * mu;
+  const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int r = pr[q][0], c = pr[q][1];
+    const double tre = 0.5 * (W.re[3 * r + c] - W.re[3 * c + r]);
+    const double tim = 0.5 * (W.im[3 * r + c] + W.im[3 * c + r]);
+    h[2 * q] = (T)((double)h[2 * q] - a.eps * tim);
+    h[2 * q + 1] = (T)((double)h[2 * q + 1] + a.eps * tre);
+  }
+  h[6] = (T)((double)h[6] - a.eps * (W.im[0] - imtr));
+  h[7] = (T)((double)h[7] - a.eps * (W.im[4] - imtr));
+}
+
+template <typename T>
+static void gauge_force(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_gauge_coeffs_t *c,
+                        double eps) {
+  using namespace std::chrono;
+  const double t0 = duration<double>(steady_clock::now().time_since_epoch()).count();
+  const Geometry &gg = ctx().g;
+  GFArgs<T> a;
+  a.u[0] = G->u[0];
+  a.u[1] = G->u[1];
+  a.h[0] = F->h[0];
+  a.h[1] = F->h[1];
+  a.g = GFGeo{{gg.nx, gg.ny, gg.nz, gg.nt}, gg.nxh, gg.vh};
+  a.c[0] = c->plaquette;
+  a.c[1] = c->rectangle;
+  a.c[2] = c->parallelogram;
+  a.eps = eps;
+  const int nlink = 8 * gg.vh;
+  hipLaunchKernelGGL(k_gauge_force<T>, dim3((nlink + 255) / 256), dim3(256), 0, ctx().stream, a);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(ctx().stream));
+  info->status = QPHIX_SUCCESS;
+  info->final_sec = duration<double>(steady_clock::now().time_since_epoch()).count() - t0;
+  info->final_flop = 61092.0;  // the reference's constant, gauge_force_imp.cpp:197
+}
+
+} // namespace qphix
+
+using namespace qphix;
+
 extern "C" {
-void QPHIX_D3_symanzik_1loop_gauge_force(QPHIX_info_t *info, QPHIX_D3_GaugeField *, QPHIX_D3_Force *,
-                                         QPHIX_gauge_coeffs_t *, double) {
-  qphix::fail(__FILE__, __LINE__, "gauge force not built yet");
-  if (info) info->status = QPHIX_FAIL;
-}
-void QPHIX_F3_symanzik_1loop_gauge_force(QPHIX_info_t *info, QPHIX_F3_GaugeField *, QPHIX_F3_Force *,
-                                         QPHIX_gauge_coeffs_t *, float) {
-  qphix::fail(__FILE__, __LINE__, "gauge force not built yet");
-  if (info) info->status = QPHIX_FAIL;
-}
-}
+
+#define QPHIX_GF_API(P, T)                                                                       \
+  void QPHIX_##P##3_symanzik_1loop_gauge_force(QPHIX_info_t *info, QPHIX_##P##3_GaugeField *G,   \
+                                               QPHIX_##P##3_Force *F,                            \
+                                               QPHIX_gauge_coeffs_t *coeffs, T eps) {            \
+    QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called: gauge force");                \
+    QPHIX_REQUIRE(info && G && F && coeffs, "symanzik_1loop_gauge_force: null argument");        \
+    info->status = QPHIX_FAIL;                                                                   \
+    QPHIX_REQUIRE(F->parity == G->parity,                                                        \
+                  "symanzik_1loop_gauge_force: force and gauge parities differ");                \
+    gauge_force<T>(info, G, F, coeffs, (double)eps);                                             \
+  }
+
+QPHIX_GF_API(D, double)
+QPHIX_GF_API(F, float)
+
+} // extern "C"

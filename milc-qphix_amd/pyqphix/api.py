@@ -95,6 +95,55 @@ class Links:
             self.h = None
 
 
+class Gauge:
+    """QPHIX_*3_GaugeField from a raw 72-real/site array (all links)."""
+
+    def __init__(self, s, raw, parity=EVENODD, prec="D"):
+        self.s, self.prec = s, prec
+        self._src = np.ascontiguousarray(raw, dtype=_NP[prec])
+        self.h = s.fn(prec, "create_G_from_raw")(ptr(self._src), parity)
+        if not self.h:
+            raise RuntimeError("create_G failed: %s" % (s.lib.last_error(),))
+
+    def extract(self, parity=EVENODD):
+        out = np.zeros(72 * self.s.V, dtype=_NP[self.prec])
+        self.s.fn(self.prec, "extract_G_to_raw")(ptr(out), self.h, parity)
+        return out
+
+    def free(self):
+        if self.h:
+            self.s.fn(self.prec, "destroy_G")(self.h)
+            self.h = None
+
+
+class Force:
+    """QPHIX_*3_Force (momentum) from a raw anti-Hermitian 72-real/site array."""
+
+    def __init__(self, s, raw, parity=EVENODD, prec="D"):
+        self.s, self.prec = s, prec
+        self._src = np.ascontiguousarray(raw, dtype=_NP[prec])
+        self.h = s.fn(prec, "create_F_from_raw")(ptr(self._src), parity)
+        if not self.h:
+            raise RuntimeError("create_F failed: %s" % (s.lib.last_error(),))
+
+    def extract(self, parity=EVENODD):
+        out = np.zeros(72 * self.s.V, dtype=_NP[self.prec])
+        self.s.fn(self.prec, "extract_F_to_raw")(ptr(out), self.h, parity)
+        return out
+
+    def free(self):
+        if self.h:
+            self.s.fn(self.prec, "destroy_F")(self.h)
+            self.h = None
+
+
+def gauge_force(G, Fm, coeffs, eps):
+    info = Info()
+    c = GaugeCoeffs(coeffs[0], coeffs[1], coeffs[2], coeffs[3] if len(coeffs) > 3 else 0.0)
+    G.s.fn(G.prec, "symanzik_1loop_gauge_force")(C.byref(info), G.h, Fm.h, C.byref(c), eps)
+    return info
+
+
 def dslash(L, out, inp, parity):
     L.s.fn(L.prec, "asqtad_dslash")(L.h, out.h, inp.h, parity)
 

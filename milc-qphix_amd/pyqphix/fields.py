@@ -205,6 +205,33 @@ def plane_wave(lat, p, colour, parity):
     return v
 
 
+def gauge_field(lat, kind, seed, eps=0.3):
+    """Plain SU(3) gauge links (no staggered phases), complex (V,4,3,3)."""
+    rng = np.random.default_rng(seed)
+    n = lat.V * 4
+    u = haar_su3(rng, n) if kind == "haar" else weak_su3(rng, n, eps)
+    return u.reshape(lat.V, 4, 3, 3)
+
+
+def antihermitian_traceless(rng, shape):
+    """Random traceless anti-Hermitian 3x3 matrices."""
+    a = rng.standard_normal(shape + (3, 3)) + 1j * rng.standard_normal(shape + (3, 3))
+    a = 0.5 * (a - dagger(a))
+    tr = np.trace(a, axis1=-2, axis2=-1) / 3.0
+    return a - tr[..., None, None] * np.eye(3)
+
+
+def symanzik_coeffs(beta, u0):
+    """Tadpole-improved one-loop Symanzik coefficients (Alford et al.,
+    Phys. Lett. B361 (1995) 87), scaled by beta/3 so that
+    S = -sum c Re Tr U_loop equals beta * sum c_i (1 - Re Tr U_loop / 3) up to a
+    constant: (plaquette, rectangle, parallelogram)."""
+    alpha_s = -4.0 * np.log(u0) / 3.06839
+    c = (1.0, -1.0 / (20.0 * u0 * u0) * (1.0 + 0.4805 * alpha_s),
+         -1.0 / (u0 * u0) * 0.03325 * alpha_s)
+    return tuple(beta / 3.0 * ci for ci in c)
+
+
 def free_eigenvalue(p, c1, c3, m2):
     """m2 + sum_mu (2 c1 sin p_mu + 2 c3 sin 3 p_mu)^2: eigenvalue of
     m2 - D_eo D_oe on an even-site plane wave for the free-field links."""

@@ -51,8 +51,34 @@ def lib():
                                                                C.c_double, vp, C.c_int, vp, vp, vp, vp]
         L.oracle_relative_residue_d.restype = C.c_double
         L.oracle_relative_residue_d.argtypes = [vp, vp]
+        L.oracle_gauge_action.restype = C.c_double
+        L.oracle_gauge_action.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int]
+        L.oracle_gauge_force.argtypes = [vp, vp, vp, C.c_double, vp, vp]
         _lib = L
     return _lib
+
+
+def gauge_action(dims, U_raw, coef, box=None, box_r=0, link_mu=-1):
+    """S(U) = -sum over geometric loops of c Re Tr U_loop; `box` restricts the
+    sum to loops starting within Chebyshev distance box_r of site box, and
+    link_mu >= 0 further to the loops that contain link (box, link_mu)."""
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    U = np.ascontiguousarray(U_raw, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    b = None if box is None else np.ascontiguousarray(box, dtype=np.int32)
+    return lib().oracle_gauge_action(_p(d), _p(U), _p(c), None if b is None else _p(b), box_r,
+                                     link_mu)
+
+
+def gauge_force(dims, U_raw, coef, eps, mom_raw):
+    """Returns (updated momentum, W) as raw 72-real/site arrays."""
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    U = np.ascontiguousarray(U_raw, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    mom = np.ascontiguousarray(mom_raw, dtype=np.float64).copy()
+    W = np.zeros_like(mom)
+    lib().oracle_gauge_force(_p(d), _p(U), _p(c), eps, _p(mom), _p(W))
+    return mom, W
 
 
 def _p(a):

@@ -32,6 +32,7 @@
  *    initialising, ks_multicg_offset.cpp:332).
  */
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -404,9 +405,235 @@ void invert_multi(const Links<T> *L, int parity_api, int niter, int max_restarts
   o->ret = iteration;
 }
 
+// ---------------------------------------------------------------------------
+// Symanzik one-loop gauge action and force, loop-centric.
+//
+// The reference's kernel bodies for this (gforce_imp_body / update_mg_body,
+// gauge_force_imp_complete_specialization.cpp:24-87) are in the absent codegen
+// repository; its orchestration (gauge_force_imp.cpp:136-369) fixes only the
+// arguments: coefficients for the plaquette, 1x2 rectangle and parallelogram
+// loops and a step eps applied to the momentum.  The oracle defines
+//   S(U) = - sum over geometric loops P of c_P Re Tr U_P     (each loop once)
+//   mom_mu(x) <- mom_mu(x) - eps * TA( W_mu(x) ),
+//   W_mu(x) = sum over loops P through link (x,mu) of c_P * (P traced from x,
+//             first step +mu),  TA(M) = (M - M^dag)/2 - Tr(M - M^dag)/6,
+// so that dS/dt under U -> exp(tX) U equals -Re Tr(X W) (pinned by finite
+// differences in tests).  Loops are enumerated geometrically per site:
+// plaquettes (mu<nu), rectangles 2x1 (long side mu, mu != nu) and the 16
+// three-dimensional six-link loops (four skew hexagons per elementary cube),
+// and each loop adds its contribution to every link it contains.
+// ---------------------------------------------------------------------------
+typedef std::complex<double> cd;
+struct M3 {
+  cd a[9];
+};
+inline M3 mmul(const M3 &x, const M3 &y) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      cd s = 0;
+      for (int k = 0; k < 3; ++k) s += x.a[3 * i + k] * y.a[3 * k + j];
+      r.a[3 * i + j] = s;
+    }
+  return r;
+}
+inline M3 madj(const M3 &x) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.a[3 * i + j] = std::conj(x.a[3 * j + i]);
+  return r;
+}
+inline M3 meye() {
+  M3 r;
+  for (int i = 0; i < 9; ++i) r.a[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  return r;
+}
+
+struct GLat {
+  int L[4];
+  long vh;
+  long raw(const int *x) const {
+    int c[4];
+    for (int d = 0; d < 4; ++d) c[d] = ((x[d] % L[d]) + L[d]) % L[d];
+    const int p = (c[0] + c[1] + c[2] + c[3]) & 1;
+    return p * vh + c[0] / 2 + (long)(L[0] / 2) * (c[1] + (long)L[1] * (c[2] + (long)L[2] * c[3]));
+  }
+};
+
+inline M3 get_link(const double *U, const GLat &g, const int *x, int mu) {
+  const double *p = U + 72 * g.raw(x) + 18 * mu;
+  M3 r;
+  for (int e = 0; e < 9; ++e) r.a[e] = cd(p[2 * e], p[2 * e + 1]);
+  return r;
+}
+
+// A loop: start vertex and signed steps (+d+1 forward, -(d+1) backward).
+struct Loop {
+  int x[4];
+  int n;
+  int step[6];
+  double c;
+};
+
+template <typename F> void for_each_loop(const GLat &g, const double *coef, F f) {
+  int x[4];
+  for (x[3] = 0; x[3] < g.L[3]; ++x[3])
+    for (x[2] = 0; x[2] < g.L[2]; ++x[2])
+      for (x[1] = 0; x[1] < g.L[1]; ++x[1])
+        for (x[0] = 0; x[0] < g.L[0]; ++x[0]) {
+          Loop l;
+          for (int d = 0; d < 4; ++d) l.x[d] = x[d];
+          for (int mu = 0; mu < 4; ++mu)
+            for (int nu = 0; nu < 4; ++nu) {
+              if (nu == mu) continue;
+              const int m = mu + 1, n = nu + 1;
+              if (mu < nu && coef[0] != 0) {
+                l.n = 4; l.c = coef[0];
+                int s[4] = {m, n, -m, -n};
+                for (int k = 0; k < 4; ++k) l.step[k] = s[k];
+                f(l);
+              }
+              if (coef[1] != 0) {
+                l.n = 6; l.c = coef[1];
+                int s[6] = {m, m, n, -m, -m, -n};
+                for (int k = 0; k < 6; ++k) l.step[k] = s[k];
+                f(l);
+              }
+            }
+          if (coef[2] == 0) continue;
+          for (int mu = 0; mu < 4; ++mu)
+            for (int nu = mu + 1; nu < 4; ++nu)
+              for (int rho = nu + 1; rho < 4; ++rho) {
+                const int m = mu + 1, n = nu + 1, r = rho + 1;
+                const int hex[4][6] = {{m, n, r, -m, -n, -r},
+                                       {m, r, n, -m, -r, -n},
+                                       {n, m, r, -n, -m, -r},
+                                       {n, -m, r, -n, m, -r}};
+                for (int h = 0; h < 4; ++h) {
+                  Loop q;
+                  for (int d = 0; d < 4; ++d) q.x[d] = x[d];
+                  if (h == 3) q.x[mu] += 1;  // the fourth hexagon starts at x + mu
+                  q.n = 6; q.c = coef[2];
+                  for (int k = 0; k < 6; ++k) q.step[k] = hex[h][k];
+                  f(q);
+                }
+              }
+        }
+}
+
+// Link matrices along a loop: M_k for step k, and the vertices v_k.
+inline void loop_links(const double *U, const GLat &g, const Loop &l, M3 *Mk, int (*v)[4]) {
+  int y[4] = {l.x[0], l.x[1], l.x[2], l.x[3]};
+  for (int k = 0; k < l.n; ++k) {
+    for (int d = 0; d < 4; ++d) v[k][d] = y[d];
+    const int s = l.step[k], d = (s > 0 ? s : -s) - 1;
+    if (s > 0) {
+      Mk[k] = get_link(U, g, y, d);
+      y[d] += 1;
+    } else {
+      y[d] -= 1;
+      Mk[k] = madj(get_link(U, g, y, d));
+    }
+  }
+}
+
+double gauge_action(const double *U, const GLat &g, const double *coef, const int *box_x,
+                    int box_r, int link_mu) {
+  double S = 0;
+  for_each_loop(g, coef, [&](const Loop &l) {
+    if (box_x) {
+      for (int d = 0; d < 4; ++d) {
+        int dd = ((l.x[d] - box_x[d]) % g.L[d] + g.L[d]) % g.L[d];
+        dd = dd > g.L[d] / 2 ? g.L[d] - dd : dd;
+        if (dd > box_r) return;
+      }
+    }
+    M3 Mk[6];
+    int v[6][4];
+    loop_links(U, g, l, Mk, v);
+    if (box_x && link_mu >= 0) {  // only loops that contain link (box_x, link_mu)
+      const long target = g.raw(box_x);
+      bool hit = false;
+      for (int k = 0; k < l.n && !hit; ++k) {
+        const int s = l.step[k], d = (s > 0 ? s : -s) - 1;
+        if (d != link_mu) continue;
+        hit = g.raw(s > 0 ? v[k] : v[(k + 1) % l.n]) == target;
+      }
+      if (!hit) return;
+    }
+    M3 P = Mk[0];
+    for (int k = 1; k < l.n; ++k) P = mmul(P, Mk[k]);
+    S -= l.c * (P.a[0].real() + P.a[4].real() + P.a[8].real());
+  });
+  return S;
+}
+
+void gauge_force_W(const double *U, const GLat &g, const double *coef, std::vector<M3> &W) {
+  W.assign(8 * g.vh * 4, M3{});
+  for_each_loop(g, coef, [&](const Loop &l) {
+    M3 Mk[6];
+    int v[6][4];
+    loop_links(U, g, l, Mk, v);
+    for (int k = 0; k < l.n; ++k) {
+      const int s = l.step[k], d = (s > 0 ? s : -s) - 1;
+      // cyclic product starting at k (forward link) or at k+1 (backward link)
+      const int k0 = s > 0 ? k : (k + 1) % l.n;
+      M3 P = Mk[k0];
+      for (int j = 1; j < l.n; ++j) P = mmul(P, Mk[(k0 + j) % l.n]);
+      long site;
+      if (s > 0) site = g.raw(v[k]);
+      else site = g.raw(v[(k + 1) % l.n]);
+      if (s < 0) P = madj(P);
+      M3 &w = W[4 * site + d];
+      for (int e = 0; e < 9; ++e) w.a[e] += l.c * P.a[e];
+    }
+  });
+}
+
 } // namespace
 
 extern "C" {
+
+/* S(U) for a full raw gauge field (72 reals/site, even sites first).
+ * coef = {plaquette, rectangle, parallelogram}.  box != NULL restricts the
+ * sum to loops starting within Chebyshev distance box_r of box (periodic);
+ * with link_mu >= 0 as well, only loops containing link (box, link_mu). */
+double oracle_gauge_action(const int *latsize, const double *U, const double *coef,
+                           const int *box, int box_r, int link_mu) {
+  GLat g;
+  for (int d = 0; d < 4; ++d) g.L[d] = latsize[d];
+  g.vh = (long)latsize[0] * latsize[1] * latsize[2] * latsize[3] / 2;
+  return gauge_action(U, g, coef, box, box_r, link_mu);
+}
+
+/* mom <- mom - eps * TA(W) for every link; mom is a raw anti-Hermitian array
+ * (72 reals/site).  Also returns W (72 reals/site) when W_out != NULL. */
+void oracle_gauge_force(const int *latsize, const double *U, const double *coef, double eps,
+                        double *mom, double *W_out) {
+  GLat g;
+  for (int d = 0; d < 4; ++d) g.L[d] = latsize[d];
+  g.vh = (long)latsize[0] * latsize[1] * latsize[2] * latsize[3] / 2;
+  std::vector<M3> W;
+  gauge_force_W(U, g, coef, W);
+  for (long l = 0; l < 8 * g.vh; ++l) {
+    const M3 &w = W[l];
+    M3 ta;
+    cd tr = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) ta.a[3 * i + j] = 0.5 * (w.a[3 * i + j] - std::conj(w.a[3 * j + i]));
+    tr = (ta.a[0] + ta.a[4] + ta.a[8]) / 3.0;
+    for (int i = 0; i < 3; ++i) ta.a[4 * i] -= tr;
+    for (int e = 0; e < 9; ++e) {
+      mom[18 * l + 2 * e] -= eps * ta.a[e].real();
+      mom[18 * l + 2 * e + 1] -= eps * ta.a[e].imag();
+      if (W_out) {
+        W_out[18 * l + 2 * e] = w.a[e].real();
+        W_out[18 * l + 2 * e + 1] = w.a[e].imag();
+      }
+    }
+  }
+}
+
 
 /* Lattice set-up; the reference's minimum-size checks qphix_misc.cpp:494-497. */
 int oracle_setup(const int *latsize) {
