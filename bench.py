@@ -102,6 +102,61 @@ def cg_compare(A, s, L64, lat, src):
     return out
 
 
+def batch_multishift(A, F, nrhs=8, nshift=12, dims=(32, 32, 32, 64)):
+    """32^3x64, 12 shifts log-spaced in [1e-4, 1e-1], 8 Gaussian sources:
+    batched mixed-precision multi-shift (FP32 iteration, FP64 reliable updates
+    and per-shift refinement) to a 1e-10 true residual per shift, against the
+    FP64 single-source multi-shift (the reference's algorithm) on one source."""
+    lat = F.Lattice(dims)
+    log(f"[bench] batched multi-shift: generating {dims} links")
+    fat, lng = F.make_links(lat, "weak", 1003, eps=0.3)
+    raw = F.raw_link_set(lat, fat, lng)
+    del fat, lng
+    sig = np.logspace(-4, -1, nshift)
+    masses = list(np.sqrt(sig) / 2)
+    resid = 1e-20
+    out = {"lattice": list(dims), "nrhs": nrhs, "nshift": nshift, "sigma": [float(x) for x in sig]}
+    with A.Session(dims) as s:
+        L = A.Links(s, *raw)
+        del raw
+        bs = []
+        for r in range(nrhs):
+            b = F.to_raw(F.gaussian_source(lat, 3000 + r))
+            b[6 * lat.vh:] = 0
+            bs.append(A.Vector(s, b, A.EVEN))
+        xs = [[A.Vector(s, None, A.EVEN) for _ in masses] for _ in bs]
+        s.lib.QPHIX_MI355X_sync()
+        t = time.perf_counter()
+        ret, info, res = A.invert_multi_rhs(L, xs, bs, masses, A.EVEN, resid, niter=20000, nrestart=2)
+        sec = time.perf_counter() - t
+        worst = max(res[r][j].final_rsq for r in range(nrhs) for j in range(nshift))
+        refine = sum(res[r][j].final_iter - ret for r in range(nrhs) for j in range(nshift))
+        out["mixed_batched"] = {"seconds": round(sec, 3), "seconds_per_source": round(sec / nrhs, 3),
+                                "multishift_steps": ret, "max_final_rsq": worst,
+                                "refinement_iters_total": int(refine), "status": info.status}
+        log(f"[bench] batched mixed multi-shift: {sec:.2f}s for {nrhs} sources x {nshift} shifts, "
+            f"{ret} steps, worst rsq {worst:.2e}")
+        x64 = [A.Vector(s, None, A.EVEN) for _ in masses]
+        t = time.perf_counter()
+        ret64, info64, res64 = A.invert_multi(L, x64, bs[0], masses, A.EVEN, resid, niter=20000,
+                                              nrestart=2)
+        sec64 = time.perf_counter() - t
+        errs = []
+        for j in range(nshift):
+            a = xs[0][j].extract()[:6 * lat.vh]
+            b = x64[j].extract()[:6 * lat.vh]
+            errs.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+        out["fp64_single_source"] = {"seconds": round(sec64, 3), "iters": ret64,
+                                     "final_rsq": res64[0].final_rsq}
+        out["max_rel_err_vs_fp64"] = max(errs)
+        out["speedup_per_source_vs_fp64"] = round(sec64 / (sec / nrhs), 2)
+        log(f"[bench] FP64 multi-shift, one source: {sec64:.2f}s {ret64} iters; "
+            f"speed-up per source {out['speedup_per_source_vs_fp64']}x, max rel err {max(errs):.1e}")
+        for o in [L] + bs + x64 + [v for row in xs for v in row]:
+            o.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +165,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the batched multi-shift measurement (prints its JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one lattice on one GPU; --gpus {args.gpus} is not "
@@ -118,6 +176,10 @@ def main():
 
     from pyqphix import api as A
     from pyqphix import fields as F
+
+    if args.only_batch:
+        print(json.dumps({"multishift_batched": batch_multishift(A, F)}), flush=True)
+        return 0
 
     lat = F.Lattice(DIMS)
     log("[bench] generating 32^4 weak-field links")
@@ -198,6 +260,8 @@ def main():
     for o in (L64, vin, vout, L32, vin32, vout32):
         o.free()
     s.close()
+    if not args.no_batch:
+        extra["multishift_batched"] = batch_multishift(A, F)
     print(json.dumps(out), flush=True)
     return 0
 

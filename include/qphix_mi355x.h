@@ -67,6 +67,29 @@ int QPHIX_D3_asqtad_invert_mixed(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad
                                  double mass, QPHIX_D3_ColorVector *out, QPHIX_D3_ColorVector *in,
                                  int sloppy_bits);
 
+/* Multi-shift CG on nrhs sources at once (ks_multicg_offset.cpp:205-689
+ * solves one source in one precision).  Solves (4 mass[j]^2 - D D) out[r*nshift+j]
+ * = in[r] for every source r and shift j, each to the squared relative true
+ * residual res_arg[r*nshift+j]->resid (res_arg[0]->resid where that is 0).
+ * Sources advance in blocks of rhs_block (1, 2, 4 or 8; 0 = automatic) through
+ * one multi-right-hand-side FP32 Dslash per half-step, with FP64 solutions and
+ * FP64 reliable updates of each source's base system; afterwards every shifted
+ * solution whose own true residual misses the target is finished by mixed-
+ * precision CG.  out is overwritten.  final_iter = multi-shift steps of that
+ * source + refinement iterations; final_restart = reliable updates.
+ * Returns the largest multi-shift step count. */
+int QPHIX_D3_asqtad_invert_multi_rhs(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad *asqtad,
+                                     QPHIX_invert_arg_t *inv_arg, QPHIX_resid_arg_t *res_arg[],
+                                     double *mass, int nshift, int nrhs,
+                                     QPHIX_D3_ColorVector *out[], QPHIX_D3_ColorVector *in[],
+                                     int rhs_block);
+
+/* Dslash on nvec (1..8) FP32 vectors in one launch: each link is read once
+ * and applied to all of them (ks_long_dslash.cpp:564-668 handles one vector).
+ * out[k] = D in[k] on the sites of `parity`, as QPHIX_F3_asqtad_dslash. */
+void QPHIX_F3_asqtad_dslash_multi(QPHIX_F3_FermionLinksAsqtad *asqtad, QPHIX_F3_ColorVector *out[],
+                                  QPHIX_F3_ColorVector *in[], int nvec, QPHIX_evenodd_t parity);
+
 /* ---- timing ----------------------------------------------------------------
  * Runs `reps` single-parity Dslash applications back to back on the library's
  * stream and returns the average kernel time in milliseconds measured with
@@ -75,6 +98,17 @@ double QPHIX_D3_asqtad_dslash_time(QPHIX_D3_FermionLinksAsqtad *L, QPHIX_D3_Colo
                                    QPHIX_D3_ColorVector *in, QPHIX_evenodd_t parity, int reps);
 double QPHIX_F3_asqtad_dslash_time(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVector *out,
                                    QPHIX_F3_ColorVector *in, QPHIX_evenodd_t parity, int reps);
+/* Average ms of the multi-vector Dslash kernel alone (vectors packed once). */
+double QPHIX_F3_asqtad_dslash_multi_time(QPHIX_F3_FermionLinksAsqtad *asqtad,
+                                         QPHIX_F3_ColorVector *out[], QPHIX_F3_ColorVector *in[],
+                                         int nvec, QPHIX_evenodd_t parity, int reps);
+/* Runs (once per lattice and Naik storage, then cached) the autotuner of the
+ * multi-RHS Dslash that the multi-vector Dslash and the multi-RHS solver use.
+ * us (optional, 8 doubles) receives microseconds per source for block sizes
+ * 1, 2, 4, 8 x {site-per-thread, (site, source)-per-thread}.  Returns
+ * 2 * best block size + (1 if the best mapping is per (site, source)), or 0
+ * on failure.  No reference counterpart (MI355X extension). */
+int QPHIX_F3_asqtad_mrhs_tune(QPHIX_F3_FermionLinksAsqtad *asqtad, double *us);
 /* Plain device-to-device copy of `bytes` bytes, `reps` times; average ms.
  * Used to calibrate achieved HBM bandwidth and profiler byte counters. */
 double QPHIX_MI355X_copy_time(size_t bytes, int reps);

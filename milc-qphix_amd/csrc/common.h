@@ -57,6 +57,8 @@ struct Geometry {
   int nxyz; // nxh*ny*nz (stride of t)
 };
 
+constexpr size_t PINNED_BYTES = 1 << 20;  // pinned host staging for solver state reads
+
 struct Context {
   bool initialised = false;
   Geometry g{};
@@ -68,11 +70,14 @@ struct Context {
   // reduction scratch (device): per-block partial sums
   double *partials = nullptr;
   int max_blocks = 0;
+  size_t partials_len = 0;   // doubles in `partials`
   // pinned host mirror for small scalar reads
   void *pinned = nullptr;
   // workspace cache for solver temporaries
   void *ws = nullptr;
   size_t ws_bytes = 0;
+  void *ws2 = nullptr;
+  size_t ws2_bytes = 0;
 };
 
 Context &ctx();
@@ -161,5 +166,13 @@ void launch_dslash_half(const LinksT<float> &L, int cb, HalfSite *out_h, float *
 
 // FP32 copy of FP64 links in the same layout (fields.hip)
 void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf);
+
+// Mixed-precision CG on one parity system (mixed.hip); out is the initial guess.
+int mixed_solve(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
+                QPHIX_resid_arg_t *res, double mass, VectorT<double> *out, VectorT<double> *in,
+                int sloppy_bits);
+
+// Second workspace, for drivers that call other solvers while holding theirs.
+void *ws2_get(size_t bytes);
 
 } // namespace qphix

@@ -4,6 +4,7 @@
 // the reference's thread pool, barriers and neighbour tables become one HIP
 // device, one stream and on-the-fly neighbour index arithmetic.
 #include "common.h"
+#include <algorithm>
 #include "qphix_mi355x.h"
 
 #include <cstring>
@@ -61,6 +62,17 @@ void *ws_get(size_t bytes) {
   return c.ws;
 }
 
+void *ws2_get(size_t bytes) {
+  Context &c = g_ctx;
+  if (bytes > c.ws2_bytes) {
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    dev_free(c.ws2);
+    c.ws2 = dev_alloc(bytes);
+    c.ws2_bytes = c.ws2 ? bytes : 0;
+  }
+  return c.ws2;
+}
+
 static QPHIX_status_t init_common(QPHIX_layout_t *layout) {
   Context &c = g_ctx;
   if (c.initialised) return QPHIX_SUCCESS;  // F and D share one lattice (C14)
@@ -115,11 +127,14 @@ static QPHIX_status_t init_common(QPHIX_layout_t *layout) {
     fail(__FILE__, __LINE__, "QPHIX_init: even_sites_on_node disagrees with latsize");
     return QPHIX_FAIL;
   }
-  // Reduction scratch: enough partials for one block per 64 sites, 4 values each.
+  // Reduction scratch: enough partials for one block per 64 sites, 4 values
+  // each, and for the multi-RHS kernels (8 sources, one 256-lane block per 32
+  // sites): vh / 4 values.
   c.max_blocks = (g.vh + 63) / 64 + 1024;
-  c.partials = (double *)dev_alloc(sizeof(double) * 4 * (size_t)c.max_blocks);
+  c.partials_len = std::max((size_t)4 * c.max_blocks, (size_t)g.vh / 4 + 1024);
+  c.partials = (double *)dev_alloc(sizeof(double) * c.partials_len);
   if (!c.partials) return QPHIX_MEM_ERROR;
-  HIP_CHECK(hipHostMalloc(&c.pinned, 1 << 16, hipHostMallocDefault));
+  HIP_CHECK(hipHostMalloc(&c.pinned, PINNED_BYTES, hipHostMallocDefault));
   c.initialised = true;
   return QPHIX_SUCCESS;
 }
@@ -130,6 +145,7 @@ static QPHIX_status_t finalize_common() {
   HIP_CHECK(hipStreamSynchronize(c.stream));
   dev_free(c.partials);
   dev_free(c.ws);
+  dev_free(c.ws2);
   if (c.pinned) HIP_CHECK(hipHostFree(c.pinned));
   HIP_CHECK(hipStreamDestroy(c.stream));
   const bool keep_abort = c.abort_on_error;

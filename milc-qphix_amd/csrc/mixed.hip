@@ -296,6 +296,13 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
   return iteration;
 }
 
+int mixed_solve(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
+                QPHIX_resid_arg_t *res, double mass, VectorT<double> *out, VectorT<double> *in,
+                int sloppy_bits) {
+  if (sloppy_bits == 16) return invert_mixed<HalfSite>(info, L, inv, res, mass, out, in);
+  return invert_mixed<float>(info, L, inv, res, mass, out, in);
+}
+
 } // namespace qphix
 
 using namespace qphix;

@@ -26,6 +26,12 @@ struct SolverState {
   int finished[MAXNDMASS];
   // mixed-precision solver scalars
   double mx_alpha, mx_beta, mx_gamma, mx_rsq, mx_rsq_old, mx_maxr, mx_delta, mx_rp;
+  int ru;      // batched multi-shift: reliable update / convergence check requested
+  int pad2_;
+  // batched multi-shift: 0 shift still iterating, 2 converged (this step's
+  // update is its last), 1 retired; sdone_at = the step it converged at
+  int sdone[MAXNDMASS];
+  int sdone_at[MAXNDMASS];
 };
 
 enum ScalarOp {
@@ -40,7 +46,8 @@ enum ScalarOp {
   SOP_MX_PAP,     // mixed CG: alpha from p.(-Mp)
   SOP_MX_RSQ,     // mixed CG: sloppy |r|^2, reliable-update test, beta
   SOP_MX_TRUE,    // mixed CG: true |b - Mx|^2 after a reliable update
-  SOP_MX_RP       // mixed CG: r.p for re-orthogonalising p after an update
+  SOP_MX_RP,      // mixed CG: r.p for re-orthogonalising p after an update
+  SOP_MSX_RSQ2    // batched mixed multi-shift: |r|^2, update request, alpha
 };
 
 // (fptype) cast of the reference: round to float when the solver is the F build.
@@ -166,6 +173,25 @@ __device__ inline void apply_scalar_op(SolverState *s, int op, const double *v) 
     s->final_rsq = v[0] / s->src_norm;
     s->iteration++;
     break;
+  case SOP_MSX_RSQ2: {
+    // As SOP_MS_RSQ2, but the step always completes (update_pm runs) and a
+    // pause for an FP64 true-residual check is requested instead of stopping:
+    // on apparent convergence, when |r| fell by delta below its maximum since
+    // the last check, or when the step budget is spent.
+    s->rsq = v[0];
+    const double r = sqrt(v[0]);
+    if (r > s->mx_maxr) s->mx_maxr = r;
+    if (s->rsq <= s->rsqstop || r < s->mx_delta * s->mx_maxr || s->iteration >= s->max_cg) s->ru = 1;
+    // A shifted system's residual is zeta_{i+1,j} r: once that meets the
+    // target the shift takes this step's update and then retires.
+    for (int j = 0; j < s->num_now; ++j)
+      if (j != s->j_low && s->sdone[j] == 0 && s->zeta_ip1[j] * s->zeta_ip1[j] * s->rsq < s->rsqstop) {
+        s->sdone[j] = 2;
+        s->sdone_at[j] = s->iteration;
+      }
+    ms_alpha_recurrence<double>(s);
+    break;
+  }
   case SOP_MX_RP:
     // p <- r + beta (p - gamma r): gamma = r.p/|r|^2 restores p's
     // orthogonality to the new residual, beta = |r_true|^2 / |r_sloppy,old|^2.

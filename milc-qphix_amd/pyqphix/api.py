@@ -148,6 +148,31 @@ def dslash(L, out, inp, parity):
     L.s.fn(L.prec, "asqtad_dslash")(L.h, out.h, inp.h, parity)
 
 
+def dslash_multi(L, outs, inps, parity, reps=0):
+    """QPHIX_F3_asqtad_dslash_multi (reps=0) or its timing variant (average ms)."""
+    assert L.prec == "F" and len(outs) == len(inps)
+    n = len(inps)
+    ho = (C.c_void_p * n)(*[v.h for v in outs])
+    hi = (C.c_void_p * n)(*[v.h for v in inps])
+    if reps:
+        return L.s.lib.QPHIX_F3_asqtad_dslash_multi_time(L.h, ho, hi, n, parity, reps)
+    L.s.lib.QPHIX_F3_asqtad_dslash_multi(L.h, ho, hi, n, parity)
+    return None
+
+
+def mrhs_tune(L):
+    """QPHIX_F3_asqtad_mrhs_tune: (best block size, per-(site, source) mapping?,
+    {(block, mapping): microseconds per source})."""
+    assert L.prec == "F"
+    us = (C.c_double * 8)()
+    code = L.s.lib.QPHIX_F3_asqtad_mrhs_tune(L.h, us)
+    if code == 0:
+        raise RuntimeError("QPHIX_F3_asqtad_mrhs_tune failed")
+    table = {(nr, m): us[2 * c + k] for c, nr in enumerate((1, 2, 4, 8))
+             for k, m in enumerate(("site", "lane")) if us[2 * c + k] > 0}
+    return code // 2, bool(code & 1), table
+
+
 def dslash_time(L, out, inp, parity, reps):
     return L.s.fn(L.prec, "asqtad_dslash_time")(L.h, out.h, inp.h, parity, reps)
 
@@ -170,6 +195,23 @@ def invert_mixed(L, out, inp, mass, parity, resid, sloppy_bits=32, niter=1000, n
     res = ResidArg(resid, 0, 0, 0, 0, 0, 0, 0)
     ret = L.s.lib.QPHIX_D3_asqtad_invert_mixed(C.byref(info), L.h, C.byref(inv), C.byref(res),
                                                mass, out.h, inp.h, sloppy_bits)
+    return ret, info, res
+
+
+def invert_multi_rhs(L, outs, inps, masses, parity, resid, niter=1000, nrestart=5, rhs_block=0):
+    """QPHIX_D3_asqtad_invert_multi_rhs: outs[r][j] for sources inps[r] and
+    masses[j]; returns (ret, info, res[r][j])."""
+    assert L.prec == "D"
+    nr, ns = len(inps), len(masses)
+    info = Info()
+    inv = InvertArg(parity, niter, 0, nrestart, parity)
+    res = [[ResidArg(resid, 0, 0, 0, 0, 0, 0, 0) for _ in masses] for _ in inps]
+    resp = (C.POINTER(ResidArg) * (nr * ns))(*[C.pointer(res[r][j]) for r in range(nr) for j in range(ns)])
+    m = (C.c_double * ns)(*masses)
+    ho = (C.c_void_p * (nr * ns))(*[outs[r][j].h for r in range(nr) for j in range(ns)])
+    hi = (C.c_void_p * nr)(*[v.h for v in inps])
+    ret = L.s.lib.QPHIX_D3_asqtad_invert_multi_rhs(C.byref(info), L.h, C.byref(inv), resp, m, ns,
+                                                   nr, ho, hi, rhs_block)
     return ret, info, res
 
 

@@ -1,0 +1,72 @@
+"""Multi-right-hand-side Dslash and batched mixed-precision multi-shift CG,
+against the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import lattice_session
+from helpers import half, random_links, rel_l2
+from pyqphix import api as A
+from pyqphix import fields as F
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8])
+def test_dslash_multi_matches_oracle(n):
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    raw = [F.to_raw(a) for a in random_links(lat, "weak", 31)]
+    orc = O.Oracle(dims, *raw)
+    L = A.Links(s, *[r.astype(np.float32) for r in raw], prec="F")
+    srcs = [F.to_raw(F.gaussian_source(lat, 100 + k), np.float32) for k in range(n)]
+    vin = [A.Vector(s, x, A.EVENODD, "F") for x in srcs]
+    vout = [A.Vector(s, None, A.EVENODD, "F") for _ in srcs]
+    for parity, cb in ((A.EVEN, 0), (A.ODD, 1)):
+        A.dslash_multi(L, vout, vin, parity)
+        for k in range(n):
+            ref = orc.dslash(half(srcs[k].astype(np.float64), lat, 1 - cb), cb)
+            assert rel_l2(half(vout[k].extract(), lat, cb), ref) < 1e-6
+    for o in vin + vout + [L]:
+        o.free()
+
+
+def true_rsq(orc, lat, x, b, mass, cb):
+    xh = half(x, lat, cb)
+    bh = half(b, lat, cb)
+    r = bh - (4 * mass * mass * xh - orc.dslash(orc.dslash(xh, 1 - cb), cb))
+    return float(r @ r / (bh @ bh))
+
+
+@pytest.mark.parametrize("nrhs,block", [(3, 0), (4, 4), (8, 8), (2, 1)])
+def test_multi_rhs_multishift(nrhs, block):
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    raw = [F.to_raw(a) for a in random_links(lat, "weak", 1003)]
+    orc = O.Oracle(dims, *raw)
+    L = A.Links(s, *raw)
+    masses = [0.05, 0.02, 0.1, 0.3]       # API masses: shifts 4 m^2
+    resid = 1e-20
+    bs = [F.to_raw(F.gaussian_source(lat, 200 + r, parity=0)) for r in range(nrhs)]
+    vb = [A.Vector(s, b, A.EVEN) for b in bs]
+    vx = [[A.Vector(s, None, A.EVEN) for _ in masses] for _ in bs]
+    print(f"[case] multi_rhs nrhs={nrhs} block={block}", flush=True)
+    ret, info, res = A.invert_multi_rhs(L, vx, vb, masses, A.EVEN, resid, niter=1000,
+                                        nrestart=2, rhs_block=block)
+    print(f"[case] ret={ret} iters={[[res[r][j].final_iter for j in range(len(masses))] for r in range(nrhs)]}",
+          flush=True)
+    assert info.status == 0 and ret > 0
+    for r in range(nrhs):
+        xo, io = orc.invert_multi(bs[r], np.array(masses), O.EVEN, resid)
+        # the batched base system converges like the oracle's (within 10%)
+        assert ret <= 1.1 * io["ret"] + 5
+        for j, m in enumerate(masses):
+            x = vx[r][j].extract()
+            assert res[r][j].final_rsq < resid
+            assert true_rsq(orc, lat, x, bs[r], m, 0) < 2 * resid
+            # two solutions at a 1e-10 relative residual differ by up to cond(M) * 1e-10
+            assert rel_l2(half(x, lat, 0), half(xo[j], lat, 0)) < 1e-5
+    for o in [L] + vb + [v for row in vx for v in row]:
+        o.free()
