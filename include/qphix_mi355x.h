@@ -67,6 +67,38 @@ int QPHIX_D3_asqtad_invert_mixed(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad
                                  double mass, QPHIX_D3_ColorVector *out, QPHIX_D3_ColorVector *in,
                                  int sloppy_bits);
 
+/* Fermion-force outer products (no reference counterpart: milc-qphix has no
+ * fermion force).  For the pseudofermion action
+ *     S = sum_k coef[k] Re phi_k^dag X_k,  X_k = (4 m_k^2 - D_eo D_oe)^{-1} phi_k,
+ * given the solutions X[k] (their even halves; e.g. the outputs of
+ * asqtad_invert_multi, several sources being more terms), writes for every
+ * site y (both parities) and direction mu, in raw MILC layout (72 reals per
+ * site, even sites first, [mu][row][col][re,im]):
+ *     oprod_fat [y][mu] = eps(y) sum_k coef[k] psi_k(y+mu)  psi_k(y)^dag
+ *     oprod_long[y][mu] = eps(y) sum_k coef[k] psi_k(y+3mu) psi_k(y)^dag
+ * where psi_k = X_k on even sites and D_oe X_k on odd sites (D from asqtad)
+ * and eps = +1 on even, -1 on odd sites.  Then, for the forward fat and long
+ * links U as passed to asqtad_create_L_from_raw (phases folded in; back links
+ * -U^dag(y-mu) as MILC double-stores them),
+ *     dS = 2 Re sum_{y,mu} Tr[dU_fat oprod_fat] + 2 Re sum Tr[dU_long oprod_long].
+ * Either output may be NULL.  Accumulation is in FP64 in both precisions.
+ * info: final_sec, final_flop, status. */
+void QPHIX_D3_asqtad_fermion_force_oprod(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad *asqtad,
+                                         int nterms, const double *coef, QPHIX_D3_ColorVector *X[],
+                                         double *oprod_fat, double *oprod_long);
+void QPHIX_F3_asqtad_fermion_force_oprod(QPHIX_info_t *info, QPHIX_F3_FermionLinksAsqtad *asqtad,
+                                         int nterms, const double *coef, QPHIX_F3_ColorVector *X[],
+                                         float *oprod_fat, float *oprod_long);
+/* Average ms of the device part of fermion_force_oprod (the D_oe X_k and the
+ * outer-product kernel; outputs kept in HBM, no export); oprod_ms (optional)
+ * receives the outer-product kernel's own average ms. */
+double QPHIX_D3_asqtad_fermion_force_time(QPHIX_D3_FermionLinksAsqtad *asqtad, int nterms,
+                                          const double *coef, QPHIX_D3_ColorVector *X[], int reps,
+                                          double *oprod_ms);
+double QPHIX_F3_asqtad_fermion_force_time(QPHIX_F3_FermionLinksAsqtad *asqtad, int nterms,
+                                          const double *coef, QPHIX_F3_ColorVector *X[], int reps,
+                                          double *oprod_ms);
+
 /* Multi-shift CG on nrhs sources at once (ks_multicg_offset.cpp:205-689
  * solves one source in one precision).  Solves (4 mass[j]^2 - D D) out[r*nshift+j]
  * = in[r] for every source r and shift j, each to the squared relative true

@@ -225,3 +225,32 @@ def invert_multi(L, outs, inp, masses, parity, resid, niter=1000, nrestart=5):
     ret = L.s.fn(L.prec, "asqtad_invert_multi")(C.byref(info), L.h, C.byref(inv), resp, m,
                                                 len(masses), hs, inp.h)
     return ret, info, res
+
+
+def fermion_force_oprod(L, coef, xs):
+    """QPHIX_*3_asqtad_fermion_force_oprod: outer products (O_fat, O_long) as
+    complex (V, 4, 3, 3) arrays in raw site order, for
+    S = sum_k coef[k] Re phi_k^dag X_k given the solutions xs[k] (Vectors
+    holding their even halves)."""
+    n = len(xs)
+    V = L.s.V
+    o1 = np.zeros(72 * V, dtype=_NP[L.prec])
+    o3 = np.zeros(72 * V, dtype=_NP[L.prec])
+    info = Info()
+    c = (C.c_double * max(n, 1))(*coef)
+    hs = (C.c_void_p * max(n, 1))(*[x.h for x in xs])
+    L.s.fn(L.prec, "asqtad_fermion_force_oprod")(C.byref(info), L.h, n, c, hs, ptr(o1), ptr(o3))
+    if info.status != 0:
+        raise RuntimeError("fermion_force_oprod failed: %s" % (L.s.lib.last_error(),))
+    cv = lambda a: a.astype(np.float64).view(np.complex128).reshape(V, 4, 3, 3)
+    return cv(o1), cv(o3), info
+
+
+def fermion_force_time(L, coef, xs, reps=10):
+    """(ms of the device part, ms of the outer-product kernel alone)."""
+    n = len(xs)
+    c = (C.c_double * n)(*coef)
+    hs = (C.c_void_p * n)(*[x.h for x in xs])
+    k = C.c_double(0.0)
+    t = L.s.fn(L.prec, "asqtad_fermion_force_time")(L.h, n, c, hs, reps, C.byref(k))
+    return t, k.value

@@ -108,6 +108,11 @@ class Lib:
             _sig(lib, f"QPHIX_{P}3_asqtad_invert_multi", C.c_int,
                  [C.POINTER(Info), _dp, C.POINTER(InvertArg), C.POINTER(C.POINTER(ResidArg)),
                   C.POINTER(R), C.c_int, C.POINTER(_dp), _dp])
+            _sig(lib, f"QPHIX_{P}3_asqtad_fermion_force_oprod", None,
+                 [C.POINTER(Info), _dp, C.c_int, C.POINTER(C.c_double), C.POINTER(_dp), _dp, _dp])
+            _sig(lib, f"QPHIX_{P}3_asqtad_fermion_force_time", C.c_double,
+                 [_dp, C.c_int, C.POINTER(C.c_double), C.POINTER(_dp), C.c_int,
+                  C.POINTER(C.c_double)])
         for n in ("QPHIX_init", "QPHIX_init_F", "QPHIX_init_D"):
             _sig(lib, n, C.c_int, [C.POINTER(Layout)])
         for n in ("QPHIX_finalize", "QPHIX_finalize_F", "QPHIX_finalize_D"):

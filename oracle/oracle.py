@@ -54,6 +54,7 @@ def lib():
         L.oracle_gauge_action.restype = C.c_double
         L.oracle_gauge_action.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int]
         L.oracle_gauge_force.argtypes = [vp, vp, vp, C.c_double, vp, vp]
+        L.oracle_fermion_oprod_d.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -145,3 +146,16 @@ class Oracle:
             self.h, parity, niter, nrestart, resid, _p(m), len(masses), C.cast(ptrs, C.c_void_p),
             _p(b), _p(od), _p(oi))
         return xs, dict(ret=ret, final_rsq=od[0], final_flop=od[1], final_iter=int(oi[0]))
+
+    def fermion_oprod(self, coef, xs):
+        """Fermion-force outer products (raw 72-real/site arrays, both parities)
+        for S = sum_k coef[k] Re phi^dag X_k given the solutions X_k (full raw
+        vectors, even half used): returns (O_fat, O_long)."""
+        assert self.p == "d"
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in xs]
+        ptrs = (C.c_void_p * len(xs))(*[x.ctypes.data for x in xs])
+        c = np.ascontiguousarray(coef, dtype=np.float64)
+        o1 = np.zeros(72 * 2 * self.vh)
+        o3 = np.zeros(72 * 2 * self.vh)
+        self.L.oracle_fermion_oprod_d(self.h, len(xs), _p(c), C.cast(ptrs, C.c_void_p), _p(o1), _p(o3))
+        return o1, o3

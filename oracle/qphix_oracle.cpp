@@ -11,6 +11,7 @@
  *   CG                             ks_d_congrad_fn.cpp:62-188 (BLAS), 221-619 (driver)
  *   multi-shift CG                 ks_multicg_offset.cpp:69-184 (BLAS), 205-689 (driver)
  *   relative residue               ks_blas_utils_c.cpp:110-149
+ *   fermion-force outer products   no reference counterpart (derived from the action)
  *
  * Layout is the reference's VECLEN=1 layout without the XY_PAD site (padding
  * never changes a result): site index ind = xh + Nxh*(y + Ny*(z + Nz*t)) within
@@ -590,6 +591,54 @@ void gauge_force_W(const double *U, const GLat &g, const double *coef, std::vect
   });
 }
 
+/* Fermion-force outer products.  No reference counterpart (milc-qphix has no
+ * fermion force); the quantity is fixed by the action
+ *   S = sum_k c_k Re phi^dag X_k,  X_k = (sigma_k - D_eo D_oe)^{-1} phi,
+ * whose derivative with respect to a forward link U(y,mu) (back link
+ * -U(y-mu)^dag, as in `dslash` with MILC's double-store) is
+ *   dS = 2 Re Tr[dU(y,mu) O(y,mu)],  O(y,mu) = eps(y) sum_k c_k psi_k(y+mu) psi_k(y)^dag,
+ * psi_k = X_k on even sites and D_oe X_k on odd ones, eps = +1 even, -1 odd;
+ * Naik links likewise with y+3mu.  The pin is the exact-difference FD of S in
+ * tests/test_oracle_force.py.  X[k] are full raw vectors (even half used);
+ * o1, o3 raw 72-real/site arrays ([mu][row][col][re,im], even sites first). */
+void fermion_oprod(const Links<double> *L, int nt, const double *c, const double *const *X,
+                   double *o1, double *o3) {
+  const long vh = G.vh;
+  std::vector<std::vector<double>> Y(nt);
+  for (int k = 0; k < nt; ++k) {
+    Y[k].assign(6 * vh, 0.0);
+    dslash(L, Y[k].data(), X[k], 1);
+  }
+  for (int cb = 0; cb < 2; ++cb) {
+    const unsigned *ng = G.neigh[cb].data();
+    const double eps = cb ? -1.0 : 1.0;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < vh; ++i)
+      for (int mu = 0; mu < 4; ++mu) {
+        double a1[18] = {0}, a3[18] = {0};
+        for (int k = 0; k < nt; ++k) {
+          const double *here = cb ? Y[k].data() + 6 * i : X[k] + 6 * i;
+          const double *other = cb ? X[k] : Y[k].data();
+          const double *f1 = other + 6 * (long)ng[16 * i + 2 * mu + 1];
+          const double *f3 = other + 6 * (long)ng[16 * i + 8 + 2 * mu + 1];
+          for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q) {
+              const double br = here[2 * q], bi = -here[2 * q + 1]; /* conj(psi(y)_q) */
+              a1[6 * r + 2 * q] += c[k] * (f1[2 * r] * br - f1[2 * r + 1] * bi);
+              a1[6 * r + 2 * q + 1] += c[k] * (f1[2 * r] * bi + f1[2 * r + 1] * br);
+              a3[6 * r + 2 * q] += c[k] * (f3[2 * r] * br - f3[2 * r + 1] * bi);
+              a3[6 * r + 2 * q + 1] += c[k] * (f3[2 * r] * bi + f3[2 * r + 1] * br);
+            }
+        }
+        const long o = 72 * (cb * vh + i) + 18 * mu;
+        for (int e = 0; e < 18; ++e) {
+          o1[o + e] = eps * a1[e];
+          o3[o + e] = eps * a3[e];
+        }
+      }
+  }
+}
+
 } // namespace
 
 extern "C" {
@@ -713,6 +762,11 @@ int oracle_invert_multi_f(void *L, int parity, int niter, int nrestart, double r
   out_d[0] = o.final_rsq; out_d[1] = o.final_flop;
   out_i[0] = o.final_iter;
   return o.ret;
+}
+
+void oracle_fermion_oprod_d(void *L, int nt, const double *c, const double *const *X, double *o1,
+                            double *o3) {
+  fermion_oprod((const Links<double> *)L, nt, c, X, o1, o3);
 }
 
 } /* extern "C" */
