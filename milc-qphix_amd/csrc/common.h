@@ -65,6 +65,7 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool sync_calls = true;    // every API call synchronises before returning
+  bool graphs = true;        // batched solvers replay captured hipGraphs
   bool abort_on_error = true;
   int n_cu = 0;
   // reduction scratch (device): per-block partial sums
@@ -97,13 +98,30 @@ void *dev_alloc(size_t bytes);
 // (slot 2mu+1) and the back link (slot 2mu) form a "pair" of NC complex numbers
 // (NC = 18, or 12 when only rows 0-1 of each long link are stored).  A pair is
 // cut into 16-byte chunks (1 complex double or 2 complex float) and stored
-// chunk-major over sites: elem(mu, chunk, i) at ((mu*NCH + chunk)*vh + i).
-// Consecutive lanes load consecutive 16 B -> every link load is a full
-// coalesced 1 KiB wavefront transaction.
+// tile-major: LINK_TILE consecutive sites form a tile, and inside a tile
+// chunk k of direction mu is a row of LINK_TILE chunks,
+//     elem(mu, k, i) at ((i / LINK_TILE) * 4*NCH + mu*NCH + k) * LINK_TILE + i % LINK_TILE.
+// Consecutive lanes load consecutive 16 B (full coalesced wavefront
+// transactions), and one wavefront's link reads are a single contiguous
+// stream (4*NCH KiB for 64 sites) instead of 4*NCH streams vh*16 B apart.
 // ---------------------------------------------------------------------------
 template <typename T> struct vec16;
 template <> struct vec16<double> { using type = double2; static constexpr int ncplx = 1; };
 template <> struct vec16<float> { using type = float4; static constexpr int ncplx = 2; };
+
+#ifndef QPHIX_LINK_TILE
+#define QPHIX_LINK_TILE 64
+#endif
+constexpr int LINK_TILE = QPHIX_LINK_TILE;
+
+// 16-byte chunk index of chunk k of direction mu at site i (nch chunks per direction).
+__host__ __device__ __forceinline__ size_t link_chunk(int mu, int k, int i, int nch) {
+  return ((size_t)(i / LINK_TILE) * (4 * nch) + mu * nch + k) * LINK_TILE + (i % LINK_TILE);
+}
+// Sites of link storage per parity (vh rounded up to whole tiles).
+__host__ __device__ __forceinline__ size_t link_sites(int vh) {
+  return (size_t)((vh + LINK_TILE - 1) / LINK_TILE) * LINK_TILE;
+}
 
 template <typename T> struct LinksT {
   T *fat[2] = {nullptr, nullptr};
@@ -156,6 +174,7 @@ struct DslashExtra {
   void *state = nullptr;       // device SolverState updated by the last block
   int scalar_op = 0;           // ScalarOp run by the last block
   const int *gate = nullptr;   // kernel is a no-op when *gate == 0
+  bool stream_out = false;     // EPI_PLAIN: non-temporal output stores (result not reused soon)
 };
 
 template <typename T>

@@ -191,6 +191,7 @@ void QPHIX_MI355X_clear_error(void) {
 }
 
 void QPHIX_MI355X_set_sync(int on) { g_ctx.sync_calls = on != 0; }
+void QPHIX_MI355X_set_graphs(int on) { g_ctx.graphs = on != 0; }
 
 void QPHIX_MI355X_sync(void) {
   if (g_ctx.initialised) HIP_CHECK(hipStreamSynchronize(g_ctx.stream));

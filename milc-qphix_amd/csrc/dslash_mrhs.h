@@ -87,10 +87,13 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs(const MrhsArgs a) 
     if constexpr (NL == 12) sg = a.lsign[i];
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) {
-      apply_pair_mrhs<18, NR>(a.fat + (size_t)mu * NCH_F * vh, vh, i, a.in, n.f1[mu], n.b1[mu], 0,
-                              0, mask, acc);
-      apply_pair_mrhs<NL, NR>(a.lng + (size_t)mu * NCH_L * vh, vh, i, a.in, n.f3[mu], n.b3[mu],
-                              (sg >> (2 * mu + 1)) & 1u, (sg >> (2 * mu)) & 1u, mask, acc);
+      const size_t tile = (size_t)(i / LINK_TILE);
+      const int ii = i % LINK_TILE;
+      apply_pair_mrhs<18, NR>(a.fat + (tile * 4 * NCH_F + mu * NCH_F) * LINK_TILE, LINK_TILE, ii,
+                              a.in, n.f1[mu], n.b1[mu], 0, 0, mask, acc);
+      apply_pair_mrhs<NL, NR>(a.lng + (tile * 4 * NCH_L + mu * NCH_L) * LINK_TILE, LINK_TILE, ii,
+                              a.in, n.f3[mu], n.b3[mu], (sg >> (2 * mu + 1)) & 1u,
+                              (sg >> (2 * mu)) & 1u, mask, acc);
     }
   }
   if constexpr (EPI == EPI_PLAIN) {
@@ -173,11 +176,13 @@ __device__ __forceinline__ void mrhs_lane_site(const MrhsArgs &a, int i, int r, 
   if constexpr (NL == 12) sg = a.lsign[i];
 #pragma unroll
   for (int mu = 0; mu < 4; ++mu) {
-    apply_pair<float, 18, float, true>(a.fat + (size_t)mu * NCH_F * vh, vh, i, a.in, n.f1[mu],
-                                       n.b1[mu], 0, 0, acc);
-    apply_pair<float, NL, float, true>(a.lng + (size_t)mu * NCH_L * vh, vh, i, a.in, n.f3[mu],
-                                       n.b3[mu], (sg >> (2 * mu + 1)) & 1u, (sg >> (2 * mu)) & 1u,
-                                       acc);
+    const size_t tile = (size_t)(i / LINK_TILE);
+    const int ii = i % LINK_TILE;
+    apply_pair<float, 18, float, true>(a.fat + (tile * 4 * NCH_F + mu * NCH_F) * LINK_TILE, LINK_TILE,
+                                       ii, a.in, n.f1[mu], n.b1[mu], 0, 0, acc);
+    apply_pair<float, NL, float, true>(a.lng + (tile * 4 * NCH_L + mu * NCH_L) * LINK_TILE, LINK_TILE,
+                                       ii, a.in, n.f3[mu], n.b3[mu], (sg >> (2 * mu + 1)) & 1u,
+                                       (sg >> (2 * mu)) & 1u, acc);
   }
 }
 

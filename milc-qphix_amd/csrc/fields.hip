@@ -31,7 +31,7 @@ __global__ void k_import_pairs(T *dst, const T *fwd, const T *back, int vh, int 
     for (int k = 0; k < nc; ++k) {
       const int kk = k < ncl ? k : k - ncl;
       const T *src = (k < ncl ? fwd : back) + 72 * (size_t)i + 18 * mu + 2 * kk;
-      const size_t chunk = (size_t)(mu * nch + k / CPV) * vh + i;
+      const size_t chunk = link_chunk(mu, k / CPV, i, nch);
       T *d = dst + chunk * (2 * CPV) + 2 * (k % CPV);
       d[0] = src[0];
       d[1] = src[1];
@@ -87,7 +87,7 @@ __global__ void k_export_fwd(const T *fat, const T *lng, const uint8_t *sign, in
   constexpr int CPV = vec16<T>::ncplx;
   for (int mu = 0; mu < 4; ++mu) {
     for (int k = 0; k < 9; ++k) {
-      const size_t chunk = (size_t)(mu * (18 / CPV) + k / CPV) * vh + i;
+      const size_t chunk = link_chunk(mu, k / CPV, i, 18 / CPV);
       const T *s = fat + chunk * (2 * CPV) + 2 * (k % CPV);
       fat_raw[72 * (size_t)i + 18 * mu + 2 * k] = s[0];
       fat_raw[72 * (size_t)i + 18 * mu + 2 * k + 1] = s[1];
@@ -95,7 +95,7 @@ __global__ void k_export_fwd(const T *fat, const T *lng, const uint8_t *sign, in
     cplx<double> w[9];
     const int nch = 2 * ncl / CPV;
     for (int k = 0; k < ncl; ++k) {
-      const size_t chunk = (size_t)(mu * nch + k / CPV) * vh + i;
+      const size_t chunk = link_chunk(mu, k / CPV, i, nch);
       const T *s = lng + chunk * (2 * CPV) + 2 * (k % CPV);
       w[k].re = s[0];
       w[k].im = s[1];
@@ -115,7 +115,7 @@ __device__ __forceinline__ cplx<double> link_elem(const T *base, int vh, int ncl
                                                   int k, int i) {
   constexpr int CPV = vec16<T>::ncplx;
   const int kk = fb ? k : ncl + k;
-  const size_t chunk = (size_t)(mu * (2 * ncl / CPV) + kk / CPV) * vh + i;
+  const size_t chunk = link_chunk(mu, kk / CPV, i, 2 * ncl / CPV);
   const T *s = base + chunk * (2 * CPV) + 2 * (kk % CPV);
   return cplx<double>{(double)s[0], (double)s[1]};
 }
@@ -169,8 +169,8 @@ __global__ void k_links_to_float(float *dst, const double *src, int vh, int nc_p
   if (i >= vh) return;
   for (int mu = 0; mu < 4; ++mu)
     for (int k = 0; k < nc_pair; ++k) {
-      const double *s = src + ((size_t)(mu * nc_pair + k) * vh + i) * 2;
-      float *d = dst + ((size_t)(mu * (nc_pair / 2) + k / 2) * vh + i) * 4 + 2 * (k % 2);
+      const double *s = src + link_chunk(mu, k, i, nc_pair) * 2;
+      float *d = dst + link_chunk(mu, k / 2, i, nc_pair / 2) * 4 + 2 * (k % 2);
       d[0] = (float)s[0];
       d[1] = (float)s[1];
     }
@@ -247,8 +247,8 @@ void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf) {
   Lf.parity = Ld.parity;
   for (int cb = 0; cb < 2; ++cb) {
     if (!Ld.fat[cb]) continue;
-    Lf.fat[cb] = (float *)dev_alloc(sizeof(float) * 4 * 36 * (size_t)vh);
-    Lf.lng[cb] = (float *)dev_alloc(sizeof(float) * 4 * 2 * Ld.recon * (size_t)vh);
+    Lf.fat[cb] = (float *)dev_alloc(sizeof(float) * 4 * 36 * link_sites(vh));
+    Lf.lng[cb] = (float *)dev_alloc(sizeof(float) * 4 * 2 * Ld.recon * link_sites(vh));
     hipLaunchKernelGGL(k_links_to_float, dim3(nblk(vh)), dim3(256), 0, ctx().stream, Lf.fat[cb],
                        Ld.fat[cb], vh, 18);
     hipLaunchKernelGGL(k_links_to_float, dim3(nblk(vh)), dim3(256), 0, ctx().stream, Lf.lng[cb],
@@ -340,8 +340,8 @@ static typename Handles<T>::L *create_L(const T *fat, const T *lng, const T *fat
     h2d(stage + nraw, fatback + off, sizeof(T) * nraw);
     h2d(stage + 2 * nraw, lng + off, sizeof(T) * nraw);
     h2d(stage + 3 * nraw, lngback + off, sizeof(T) * nraw);
-    L->fat[cb] = (T *)dev_alloc(sizeof(T) * 144 * (size_t)vh);
-    L->lng[cb] = (T *)dev_alloc(sizeof(T) * 2 * 2 * ncl * 4 * (size_t)vh);
+    L->fat[cb] = (T *)dev_alloc(sizeof(T) * 144 * link_sites(vh));
+    L->lng[cb] = (T *)dev_alloc(sizeof(T) * 2 * 2 * ncl * 4 * link_sites(vh));
     hipLaunchKernelGGL(k_import_pairs<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, L->fat[cb],
                        stage, stage + nraw, vh, 9);
     hipLaunchKernelGGL(k_import_pairs<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, L->lng[cb],

@@ -102,10 +102,11 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
 
   const float *plow = p + (size_t)j_low * nvr;
   const int batch = 16;
-  while (true) {
-    bool any = false;
-    for (int r = 0; r < nrhs; ++r) any |= !done[r];
-    if (!any) break;
+  // One batch of steps: kernels only (no memsets or copies), every argument
+  // fixed for this chunk, every scalar read from the device states -- so it is
+  // captured once as a hipGraph and replayed; a step whose source is paused or
+  // finished is a no-op through its gate.
+  auto enqueue_batch = [&]() {
     for (int k = 0; k < batch; ++k) {
       mrhs_dslash<NR>(Lf, other, tmp, plow, EPI_PLAIN, 0.f, nullptr, st, lane);
       mrhs_dslash<NR>(Lf, parity, Ap, tmp, EPI_PKP, (float)-offset_low, plow, st, lane);
@@ -119,6 +120,23 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
                          ctx().stream, xs, p, rs, vh, nshift, st);
       hipLaunchKernelGGL(k_msx_commit, dim3(1), dim3(NR), 0, ctx().stream, st, NR);
     }
+  };
+  hipGraphExec_t gexec = nullptr;
+  if (ctx().graphs) {
+    (void)mrhs_blocks<NR>(vh, lane, Lf.recon, EPI_PLAIN);  // grid queries happen before capture
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(ctx().stream, hipStreamCaptureModeThreadLocal));
+    enqueue_batch();
+    HIP_CHECK(hipStreamEndCapture(ctx().stream, &graph));
+    HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(graph));
+  }
+  while (true) {
+    bool any = false;
+    for (int r = 0; r < nrhs; ++r) any |= !done[r];
+    if (!any) break;
+    if (gexec) HIP_CHECK(hipGraphLaunch(gexec, ctx().stream));
+    else enqueue_batch();
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(ctx().pinned, st, NR * sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
     HIP_CHECK(hipStreamSynchronize(ctx().stream));
@@ -160,6 +178,7 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     }
   }
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
+  if (gexec) HIP_CHECK(hipGraphExecDestroy(gexec));
   return true;
 }
 

@@ -59,6 +59,7 @@ template <typename T> static void dslash_api(LinksT<T> *L, VectorT<T> *out, Vect
   QPHIX_REQUIRE(out->half[cb] && in->half[1 - cb],
                 "asqtad_dslash: out must hold the output parity and in the opposite one");
   DslashExtra ex;
+  ex.stream_out = true;  // the result goes back to the caller
   launch_dslash<T>(*L, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
   sync_if_needed();
 }
@@ -74,6 +75,7 @@ static double dslash_time(LinksT<T> *L, VectorT<T> *out, VectorT<T> *in, int par
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
   DslashExtra ex;
+  ex.stream_out = true;  // times the stand-alone asqtad_dslash
   HIP_CHECK(hipEventRecord(e0, ctx().stream));
   for (int r = 0; r < reps; ++r) launch_dslash<T>(*L, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
   HIP_CHECK(hipEventRecord(e1, ctx().stream));
