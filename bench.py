@@ -157,6 +157,141 @@ def batch_multishift(A, F, nrhs=8, nshift=12, dims=(32, 32, 32, 64)):
     return out
 
 
+def force_24(A, F, dims=(24, 24, 24, 24)):
+    """24^4, FP64: the Symanzik gauge force (beta = 6.0, u0 = 0.86) on weak-field
+    SU(3) links, and the fermion-force outer products of
+    phi^dag (M^dag M + sigma_k)^{-1} phi for 4 shifts from the FP64 multi-shift
+    solutions.  Times are device times with the fields resident in HBM."""
+    lat = F.Lattice(dims)
+    out = {"lattice": list(dims)}
+    log(f"[bench] forces at {dims}: generating fields")
+    with A.Session(dims) as s:
+        U = F.gauge_field(lat, "weak", 31, eps=0.3)
+        G = A.Gauge(s, F.to_raw(U))
+        del U
+        Fm = A.Force(s, np.zeros(72 * lat.V))
+        coef = F.symanzik_coeffs(6.0, 0.86)
+        A.gauge_force(G, Fm, coef, 0.02)
+        reps = 5
+        t = time.perf_counter()
+        for _ in range(reps):
+            A.gauge_force(G, Fm, coef, 0.02)
+        sec = (time.perf_counter() - t) / reps
+        # naive count: 181 SU(3) products (216 flops) per link for 6 + 18 + 24 loops
+        flop = 181 * 216.0 * 4 * lat.V
+        out["gauge_force"] = {"ms": round(sec * 1e3, 3), "GFLOPs": round(flop / sec / 1e9, 1),
+                              "frac_fp64_peak": round(flop / sec / 78.6e12, 4),
+                              "flop_per_link": 181 * 216}
+        log(f"[bench] gauge force {sec * 1e3:.2f} ms, {flop / sec / 1e9:.0f} GFLOP/s")
+        G.free()
+        Fm.free()
+
+        fat, lng = F.make_links(lat, "weak", 41, eps=0.3)
+        L = A.Links(s, *F.raw_link_set(lat, fat, lng))
+        del fat, lng
+        phi = F.to_raw(F.gaussian_source(lat, 43, parity=0))
+        vb = A.Vector(s, phi, A.EVEN)
+        masses = [0.05, 0.1, 0.2, 0.4]
+        fcoef = [0.5, 0.25, 0.125, 0.0625]
+        xs = [A.Vector(s, None, A.EVEN) for _ in masses]
+        ret, info, res = A.invert_multi(L, xs, vb, masses, A.EVEN, 1e-20, niter=4000)
+        t_dev, t_k = A.fermion_force_time(L, fcoef, xs, reps=10)
+        nbytes = (len(masses) * 96 + 2304) * lat.vh
+        out["fermion_force"] = {"terms": len(masses), "device_ms": round(t_dev, 4),
+                                "oprod_kernel_ms": round(t_k, 4),
+                                "oprod_bytes": nbytes,
+                                "oprod_GBs": round(nbytes / (t_k * 1e-3) / 1e9, 1),
+                                "solve_iters": ret}
+        log(f"[bench] fermion force: {t_dev:.3f} ms device ({t_k:.3f} ms outer-product kernel, "
+            f"{nbytes / (t_k * 1e-3) / 1e9:.0f} GB/s)")
+        for o in [L, vb] + xs:
+            o.free()
+    return out
+
+
+def _weak_links_gpu(lat, F, seed, eps):
+    """Forward fat (1 + 0.3 Gaussian) and Naik (exp(i eps H)) links with phases,
+    for lattices too large for the host generator: the matrix exponentials run
+    with torch on the GPU in chunks.  Synthetic input only."""
+    import torch
+    rng = np.random.default_rng(seed)
+    n = lat.V * 4
+    fat = F.near_identity(rng, n, 0.3).reshape(lat.V, 4, 3, 3)
+    lng = np.empty((n, 3, 3), dtype=np.complex128)
+    chunk = 1 << 22
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        z = torch.randn((b - a, 3, 3), dtype=torch.complex128, device="cuda", generator=g)
+        h = 0.5 * (z + z.conj().transpose(-1, -2))
+        tr = torch.diagonal(h, dim1=-2, dim2=-1).sum(-1).real / 3.0
+        h = h - tr[:, None, None] * torch.eye(3, dtype=torch.complex128, device="cuda")
+        lng[a:b] = torch.linalg.matrix_exp(1j * eps * h).cpu().numpy()
+        if (a // chunk) % 8 == 0:
+            log(f"[bench]   links {b}/{n}")
+    lng = lng.reshape(lat.V, 4, 3, 3)
+    return F.fold_phases(lat, fat, 1), F.fold_phases(lat, lng, 3)
+
+
+def combined_48(A, F, dims=(48, 48, 48, 96), nrhs=4, nshift=16, mass=0.0102):
+    """One GPU's share of a production lattice: 48^3x96 weak-field links, 16-shift
+    mixed multi-shift on 4 Gaussian sources (FP64 reliable updates, 1e-10 per
+    shift), then the fermion-force outer products of all 64 solutions; against
+    the FP64 path (FP64 multi-shift per source + the same force)."""
+    lat = F.Lattice(dims)
+    log(f"[bench] combined {dims}: generating links")
+    fat, lng = _weak_links_gpu(lat, F, 51, 0.3)
+    raw = F.raw_link_set(lat, fat, lng)
+    del fat, lng
+    sig = np.logspace(np.log10(4 * mass * mass), 0.0, nshift)
+    masses = list(np.sqrt(sig) / 2)
+    coef = list(1.0 / (1.0 + np.arange(nshift)))
+    out = {"lattice": list(dims), "nrhs": nrhs, "nshift": nshift, "mass": mass,
+           "sigma": [float(x) for x in sig]}
+    with A.Session(dims) as s:
+        L = A.Links(s, *raw)
+        del raw
+        bs = [A.Vector(s, F.to_raw(F.gaussian_source(lat, 5000 + r, parity=0)), A.EVEN)
+              for r in range(nrhs)]
+        log("[bench] combined: mixed batched multi-shift")
+        xm = [[A.Vector(s, None, A.EVEN) for _ in masses] for _ in bs]
+        s.lib.QPHIX_MI355X_sync()
+        t = time.perf_counter()
+        ret, info, res = A.invert_multi_rhs(L, xm, bs, masses, A.EVEN, 1e-20, niter=20000, nrestart=2)
+        t_solve = time.perf_counter() - t
+        flat = [x for row in xm for x in row]
+        cf = [c for _ in bs for c in coef]
+        t_force, _ = A.fermion_force_time(L, cf, flat, reps=1)
+        worst = max(r.final_rsq for row in res for r in row)
+        out["mixed"] = {"solve_s": round(t_solve, 3), "force_ms": round(t_force, 3),
+                        "end_to_end_s": round(t_solve + t_force * 1e-3, 3), "steps": ret,
+                        "worst_final_rsq": worst, "status": info.status}
+        log(f"[bench] combined mixed: solve {t_solve:.2f}s, force {t_force:.1f} ms, worst rsq {worst:.2e}")
+        o1m, o3m, _ = A.fermion_force_oprod(L, cf, flat)
+        for x in flat:
+            x.free()
+        log("[bench] combined: FP64 multi-shift per source")
+        x64 = [[A.Vector(s, None, A.EVEN) for _ in masses] for _ in bs]
+        t = time.perf_counter()
+        for b, row in zip(bs, x64):
+            A.invert_multi(L, row, b, masses, A.EVEN, 1e-20, niter=20000, nrestart=2)
+        t64 = time.perf_counter() - t
+        flat = [x for row in x64 for x in row]
+        t_force64, _ = A.fermion_force_time(L, cf, flat, reps=1)
+        o1d, o3d, _ = A.fermion_force_oprod(L, cf, flat)
+        out["fp64"] = {"solve_s": round(t64, 3), "force_ms": round(t_force64, 3),
+                       "end_to_end_s": round(t64 + t_force64 * 1e-3, 3)}
+        err1 = float(np.linalg.norm(o1m - o1d) / np.linalg.norm(o1d))
+        err3 = float(np.linalg.norm(o3m - o3d) / np.linalg.norm(o3d))
+        out["force_rel_err_mixed_vs_fp64"] = {"fat": err1, "long": err3}
+        out["speedup_end_to_end"] = round(out["fp64"]["end_to_end_s"] / out["mixed"]["end_to_end_s"], 2)
+        log(f"[bench] combined FP64: solve {t64:.2f}s; force rel err fat {err1:.1e} long {err3:.1e}; "
+            f"speed-up {out['speedup_end_to_end']}x")
+        for x in flat + [L] + bs:
+            x.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +303,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched multi-shift measurement (prints its JSON)")
+    ap.add_argument("--no-force", action="store_true")
+    ap.add_argument("--combined", action="store_true",
+                    help="run only the 48^3x96 multi-shift + fermion-force case (prints its JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one lattice on one GPU; --gpus {args.gpus} is not "
@@ -179,6 +317,9 @@ def main():
 
     if args.only_batch:
         print(json.dumps({"multishift_batched": batch_multishift(A, F)}), flush=True)
+        return 0
+    if args.combined:
+        print(json.dumps({"combined_48x96": combined_48(A, F)}), flush=True)
         return 0
 
     lat = F.Lattice(DIMS)
@@ -262,6 +403,8 @@ def main():
     s.close()
     if not args.no_batch:
         extra["multishift_batched"] = batch_multishift(A, F)
+    if not args.no_force:
+        extra["forces_24"] = force_24(A, F)
     print(json.dumps(out), flush=True)
     return 0
 

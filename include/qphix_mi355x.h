@@ -140,11 +140,11 @@ double QPHIX_F3_asqtad_dslash_multi_time(QPHIX_F3_FermionLinksAsqtad *asqtad,
                                          int nvec, QPHIX_evenodd_t parity, int reps);
 /* Runs (once per lattice and Naik storage, then cached) the autotuner of the
  * multi-RHS Dslash that the multi-vector Dslash and the multi-RHS solver use.
- * us (optional, 8 doubles) receives microseconds per source for block sizes
- * 1, 2, 4, 8 x {site-per-thread, (site, source)-per-thread}.  Returns
- * 2 * best block size + (1 if the best mapping is per (site, source)), or 0
- * on failure.  No reference counterpart (MI355X extension). */
-int QPHIX_F3_asqtad_mrhs_tune(QPHIX_F3_FermionLinksAsqtad *asqtad, double *us);
+ * us (optional, 12 doubles) receives microseconds per source for block sizes
+ * NR = 1, 2, 4, 8 x sources per work-item K = 1, 2, 4 (0 where K > NR);
+ * best_k (optional) the best K of the best block size.  Returns the best
+ * block size, or 0 on failure.  No reference counterpart (MI355X extension). */
+int QPHIX_F3_asqtad_mrhs_tune(QPHIX_F3_FermionLinksAsqtad *asqtad, double *us, int *best_k);
 /* Plain device-to-device copy of `bytes` bytes, `reps` times; average ms.
  * Used to calibrate achieved HBM bandwidth and profiler byte counters. */
 double QPHIX_MI355X_copy_time(size_t bytes, int reps);

@@ -1,30 +1,36 @@
 // dslash_mrhs.h -- Dslash applied to NR right-hand sides in one launch.
 //
-// Links are loaded once per site and applied to NR spinors, so link traffic
-// (96% of a single-RHS Dslash's bytes) is shared: bytes per site become
-// links + NR x 2 x 24 B instead of NR x (links + 48 B).  Spinors of the NR
-// right-hand sides are interleaved per site ([site][rhs][colour][re,im], FP32),
-// so one neighbour gather fetches all NR of them contiguously.  Each RHS
-// carries its own gate (its solver state's `active`); inactive RHS skip their
-// stores.  Two thread mappings: one work-item per site looping over the NR
-// sources (k_dslash_mrhs), or one per (site, source) (k_dslash_mrhs_lane).
+// Links are read from HBM once per site and applied to NR spinors, so link
+// traffic (96% of a single-RHS Dslash's bytes) is shared: bytes per site become
+// links + NR x 2 x spinor instead of NR x (links + 2 x spinor).  Spinors of
+// the NR right-hand sides are interleaved per site ([site][rhs][colour][re,im],
+// FP32 or FP64), so one neighbour gather fetches a site's sources
+// contiguously.  Each RHS carries its own gate (its solver state's `active`);
+// inactive RHS are not stored.
+//
+// Thread mapping: a work-item handles K of the NR sources of one site, so
+// G = NR / K lanes share a site.  Links reach each lane's registers once per
+// K sources: L1-to-register traffic per (site, source) is links / K + the
+// gathered spinors, while registers grow with K (K accumulators and K
+// gathers in flight).  K = 1 is one lane per (site, source); the autotuner
+// picks NR and K per lattice.
 #pragma once
 #include "dslash_kernel.h"
 
 namespace qphix {
 
-struct MrhsArgs {
-  const float4 *fat, *lng;
+template <typename T> struct MrhsArgsT {
+  const typename vec16<T>::type *fat, *lng;
   const uint8_t *lsign;
-  const float *in;   // [vh][NR][6]
-  float *out;        // [vh][NR][6]
-  const float *aux;  // EPI_PKP: p, [vh][NR][6]
+  const T *in;       // [vh][NR][6]
+  T *out;            // [vh][NR][6]
+  const T *aux;      // EPI_PKP: p, [vh][NR][6]
   Geo g;
   int cb;
-  float s;           // EPI_PKP: out = D in + s * p
+  T s;               // EPI_PKP: out = D in + s * p
   double *partials;  // EPI_PKP: [NR][gridDim.x]
   const SolverState *states;  // per-RHS gates (states[r].active)
-  int nlog;          // lane mapping: logical 256-item blocks (vh * NR / 256, rounded up)
+  int nlog;          // logical 256-item blocks (vh * NR / K / 256, rounded up)
 };
 
 template <int NR> __device__ __forceinline__ unsigned rhs_mask(const SolverState *st) {
@@ -34,113 +40,10 @@ template <int NR> __device__ __forceinline__ unsigned rhs_mask(const SolverState
   return m;
 }
 
-template <int NC, int NR>
-__device__ __forceinline__ void apply_pair_mrhs(const float4 *base, int vh, int i, const float *in,
-                                                int nf, int nb, unsigned sg_f, unsigned sg_b,
-                                                unsigned mask, cplx<float> (*acc)[3]) {
-  cplx<float> u[NC];
-  PairLoad<float, NC, true>::load(base, vh, i, u);
-  cplx<float> wf[9], wb[9];
-  if constexpr (NC == 18) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      wf[k] = u[k];
-      wb[k] = u[9 + k];
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      wf[k] = u[k];
-      wb[k] = u[6 + k];
-    }
-    recon_row2(wf, sg_f ? -1.f : 1.f);
-    recon_row2(wb, sg_b ? -1.f : 1.f);
-  }
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    if (!((mask >> r) & 1u)) continue;
-    cplx<float> vf[3], vb[3];
-    Spin<float>::load(in, nf * NR + r, vf);
-    Spin<float>::load(in, nb * NR + r, vb);
-    matvec_acc(wf, vf, acc[r]);
-    matvec_acc(wb, vb, acc[r]);
-  }
-}
-
-template <int NL, int NR, int EPI>
-__global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs(const MrhsArgs a) {
-  const unsigned mask = rhs_mask<NR>(a.states);
-  if (mask == 0) return;  // uniform over the grid
-  constexpr int NCH_F = 9, NCH_L = NL / 2;
-  const int vh = a.g.vh;
-  const int i = block_slot(OPT_XCD) * blockDim.x + threadIdx.x;
-  const bool valid = i < vh;
-  cplx<float> acc[NR][3];
-#pragma unroll
-  for (int r = 0; r < NR; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) acc[r][c] = cplx<float>{0.f, 0.f};
-  if (valid) {
-    SiteNbr n;
-    site_neighbours(a.g, i, a.cb, n);
-    unsigned sg = 0;
-    if constexpr (NL == 12) sg = a.lsign[i];
-#pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-      const size_t tile = (size_t)(i / LINK_TILE);
-      const int ii = i % LINK_TILE;
-      apply_pair_mrhs<18, NR>(a.fat + (tile * 4 * NCH_F + mu * NCH_F) * LINK_TILE, LINK_TILE, ii,
-                              a.in, n.f1[mu], n.b1[mu], 0, 0, mask, acc);
-      apply_pair_mrhs<NL, NR>(a.lng + (tile * 4 * NCH_L + mu * NCH_L) * LINK_TILE, LINK_TILE, ii,
-                              a.in, n.f3[mu], n.b3[mu], (sg >> (2 * mu + 1)) & 1u,
-                              (sg >> (2 * mu)) & 1u, mask, acc);
-    }
-  }
-  if constexpr (EPI == EPI_PLAIN) {
-    if (valid) {
-#pragma unroll
-      for (int r = 0; r < NR; ++r)
-        if ((mask >> r) & 1u) Spin<float>::store(a.out, i * NR + r, acc[r]);
-    }
-  } else {
-    double v[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) v[r] = 0.0;
-    if (valid) {
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        if (!((mask >> r) & 1u)) continue;
-        cplx<float> p[3];
-        Spin<float>::load(a.aux, i * NR + r, p);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          acc[r][c].re = acc[r][c].re + a.s * p[c].re;
-          acc[r][c].im = acc[r][c].im + a.s * p[c].im;
-          v[r] += (double)p[c].re * acc[r][c].re + (double)p[c].im * acc[r][c].im;
-        }
-        Spin<float>::store(a.out, i * NR + r, acc[r]);
-      }
-    }
-    block_sum<NR>(v);
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int r = 0; r < NR; ++r) a.partials[(size_t)r * gridDim.x + blockIdx.x] = v[r];
-    }
-  }
-}
-
-
-// Lane-per-(site, source) variant: work-item t handles site t / NR for source
-// t % NR, so a wavefront covers 64 / NR sites.  The NR lanes of a site load
-// the same link addresses (merged by the memory pipeline, so HBM link traffic
-// is still shared NR ways) while each lane keeps the register footprint of
-// a single-source Dslash; inactive sources are masked per lane and all loads
-// of an active lane are issued together, as in k_dslash.
-//
 // The grid is resident-sized (a multiple of N_XCD): the nlog logical 256-item
 // blocks are split into N_XCD contiguous slabs, XCD x = blockIdx % N_XCD walks
 // slab x with its gridDim.x / N_XCD blocks (neighbour gathers stay in that
-// XCD's L2), and EPI_PKP leaves one partial per resident block and source.
+// XCD's L2), and reducing epilogues leave one partial per resident block.
 struct XcdWalk {
   int q, per, span, base, nlog;
   __device__ XcdWalk(int nlog_) : nlog(nlog_) {
@@ -159,83 +62,132 @@ struct XcdWalk {
   }
 };
 
-template <int NL, int NR>
-__device__ __forceinline__ void mrhs_lane_site(const MrhsArgs &a, int i, int r, cplx<float> *acc) {
-  constexpr int NCH_F = 9, NCH_L = NL / 2;
-  const int vh = a.g.vh;
-  SiteNbr n;
-  site_neighbours(a.g, i, a.cb, n);
+// One direction's link pair applied to K sources (r0 .. r0+K-1) of the
+// neighbours nf (forward) and nb (back).
+template <typename T, int NC, int NR, int K>
+__device__ __forceinline__ void apply_pair_k(const typename vec16<T>::type *base, int ii, const T *in,
+                                             int nf, int nb, int r0, unsigned sg_f, unsigned sg_b,
+                                             cplx<T> (*acc)[3]) {
+  cplx<T> u[NC];
+  PairLoad<T, NC, true>::load(base, LINK_TILE, ii, u);
+  cplx<T> vf[K][3], vb[K][3];
 #pragma unroll
-  for (int mu = 0; mu < 4; ++mu) {
-    n.f1[mu] = n.f1[mu] * NR + r;
-    n.b1[mu] = n.b1[mu] * NR + r;
-    n.f3[mu] = n.f3[mu] * NR + r;
-    n.b3[mu] = n.b3[mu] * NR + r;
+  for (int k = 0; k < K; ++k) {
+    Spin<T>::load(in, nf * NR + r0 + k, vf[k]);
+    Spin<T>::load(in, nb * NR + r0 + k, vb[k]);
   }
-  unsigned sg = 0;
-  if constexpr (NL == 12) sg = a.lsign[i];
+  if constexpr (NC == 18) {
 #pragma unroll
-  for (int mu = 0; mu < 4; ++mu) {
-    const size_t tile = (size_t)(i / LINK_TILE);
-    const int ii = i % LINK_TILE;
-    apply_pair<float, 18, float, true>(a.fat + (tile * 4 * NCH_F + mu * NCH_F) * LINK_TILE, LINK_TILE,
-                                       ii, a.in, n.f1[mu], n.b1[mu], 0, 0, acc);
-    apply_pair<float, NL, float, true>(a.lng + (tile * 4 * NCH_L + mu * NCH_L) * LINK_TILE, LINK_TILE,
-                                       ii, a.in, n.f3[mu], n.b3[mu], (sg >> (2 * mu + 1)) & 1u,
-                                       (sg >> (2 * mu)) & 1u, acc);
+    for (int k = 0; k < K; ++k) {
+      matvec_acc(u, vf[k], acc[k]);
+      matvec_acc(u + 9, vb[k], acc[k]);
+    }
+  } else {
+    cplx<T> w[9];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) w[e] = u[e];
+    recon_row2(w, sg_f ? (T)-1 : (T)1);
+#pragma unroll
+    for (int k = 0; k < K; ++k) matvec_acc(w, vf[k], acc[k]);
+#pragma unroll
+    for (int e = 0; e < 6; ++e) w[e] = u[6 + e];
+    recon_row2(w, sg_b ? (T)-1 : (T)1);
+#pragma unroll
+    for (int k = 0; k < K; ++k) matvec_acc(w, vb[k], acc[k]);
   }
 }
 
-// Per-source sums of v over the block (lanes l and l + k*NR of a wavefront
-// share a source); thread r < NR returns source r's sum.
-template <int NR, int BLOCK> __device__ __forceinline__ double lane_block_sum(double v) {
+// Per-source sums over the block: lane l holds K values of source group
+// l % G (lanes l and l + m*G share a group).  Thread q < G*K returns the sum
+// for source q = group*K + k.
+template <int G, int K, int BLOCK> __device__ __forceinline__ double group_block_sum(double (&v)[K]) {
 #pragma unroll
-  for (int off = 32; off >= NR; off >>= 1) v += __shfl_down(v, off, 64);
-  __shared__ double sh[BLOCK / 64][NR];
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int off = 32; off >= G; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  __shared__ double sh[BLOCK / 64][G * K];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane < NR) sh[w][lane] = v;
+  if (lane < G) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[w][lane * K + k] = v[k];
+  }
   __syncthreads();
   double sum = 0.0;
-  if (threadIdx.x < NR) {
+  if (threadIdx.x < G * K) {
 #pragma unroll
-    for (int k = 0; k < BLOCK / 64; ++k) sum += sh[k][threadIdx.x];
+    for (int m = 0; m < BLOCK / 64; ++m) sum += sh[m][threadIdx.x];
   }
   return sum;
 }
 
-template <int NL, int NR, int EPI>
-__global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lane(const MrhsArgs a) {
+template <typename T, int NL, int NR, int K, int EPI>
+__global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs(const MrhsArgsT<T> a) {
+  static_assert(NR % K == 0, "K must divide NR");
+  constexpr int G = NR / K;
+  constexpr int CPV = vec16<T>::ncplx;
+  constexpr int NCH_F = 18 / CPV, NCH_L = NL / CPV;
   const unsigned mask = rhs_mask<NR>(a.states);
   if (mask == 0) return;  // uniform over the grid
   const int vh = a.g.vh;
   const XcdWalk wk(a.nlog);
-  double v = 0.0;
-  for (int k = 0;; ++k) {
-    const int blk = wk.at(k);
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  for (int step = 0;; ++step) {
+    const int blk = wk.at(step);
     if (blk < 0) break;
     const int t = blk * DSLASH_BLOCK + threadIdx.x;
-    const int i = t / NR, r = t % NR;
-    if (i >= vh || !((mask >> r) & 1u)) continue;
-    cplx<float> acc[3];
+    const int i = t / G, r0 = (t % G) * K;
+    const unsigned m = (mask >> r0) & ((1u << K) - 1u);
+    if (i >= vh || m == 0) continue;
+    cplx<T> acc[K][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) acc[c] = cplx<float>{0.f, 0.f};
-    mrhs_lane_site<NL, NR>(a, i, r, acc);
-    if constexpr (EPI == EPI_PKP) {
-      cplx<float> p[3];
-      Spin<float>::load(a.aux, t, p);
+    for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        acc[c].re = acc[c].re + a.s * p[c].re;
-        acc[c].im = acc[c].im + a.s * p[c].im;
-        v += (double)p[c].re * acc[c].re + (double)p[c].im * acc[c].im;
-      }
+      for (int c = 0; c < 3; ++c) acc[k][c] = cplx<T>{0, 0};
+    SiteNbr n;
+    site_neighbours(a.g, i, a.cb, n);
+    unsigned sg = 0;
+    if constexpr (NL == 12) sg = a.lsign[i];
+    const size_t tile = (size_t)(i / LINK_TILE);
+    const int ii = i % LINK_TILE;
+    // With K > 1 an unrolled direction loop lets the scheduler hoist all 16 x K
+    // gathers and overflow the register file; one direction at a time keeps
+    // 2 link pairs + 4 K gathers in flight.
+#pragma unroll(K == 1 ? 4 : 1)
+    for (int mu = 0; mu < 4; ++mu) {
+      apply_pair_k<T, 18, NR, K>(a.fat + (tile * 4 * NCH_F + mu * NCH_F) * LINK_TILE, ii, a.in, n.f1[mu],
+                                 n.b1[mu], r0, 0, 0, acc);
+      apply_pair_k<T, NL, NR, K>(a.lng + (tile * 4 * NCH_L + mu * NCH_L) * LINK_TILE, ii, a.in, n.f3[mu],
+                                 n.b3[mu], r0, (sg >> (2 * mu + 1)) & 1u, (sg >> (2 * mu)) & 1u, acc);
     }
-    Spin<float>::store(a.out, t, acc);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!((m >> k) & 1u)) continue;
+      const int slot = i * NR + r0 + k;
+      if constexpr (EPI == EPI_PKP) {
+        cplx<T> p[3];
+        Spin<T>::load(a.aux, slot, p);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          acc[k][c].re = acc[k][c].re + a.s * p[c].re;
+          acc[k][c].im = acc[k][c].im + a.s * p[c].im;
+          v[k] += (double)p[c].re * acc[k][c].re + (double)p[c].im * acc[k][c].im;
+        }
+      }
+      Spin<T>::store(a.out, slot, acc[k]);
+    }
   }
   if constexpr (EPI == EPI_PKP) {
-    const double sum = lane_block_sum<NR, DSLASH_BLOCK>(v);
+    const double sum = group_block_sum<G, K, DSLASH_BLOCK>(v);
     if (threadIdx.x < NR) a.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sum;
   }
+}
+
+// Per-source block sums for one-work-item-per-(site, source) kernels (K = 1).
+template <int NR, int BLOCK> __device__ __forceinline__ double lane_block_sum(double v) {
+  double w[1] = {v};
+  return group_block_sum<NR, 1, BLOCK>(w);
 }
 
 } // namespace qphix

@@ -1,5 +1,6 @@
-// multishift_batch.hip -- multi-shift CG on several right-hand sides at once,
-// FP32 iteration with FP64 reliable updates and per-shift FP64 refinement.
+// multishift_batch.hip -- multi-shift CG on several right-hand sides at once:
+// an FP64 first phase, then FP32 iteration with FP64 reliable updates, then
+// per-shift FP64-checked refinement.
 //
 // No counterpart in the reference, whose multi-shift solver
 // (ks_multicg_offset.cpp:205-689) takes one source in one precision.  Per
@@ -9,17 +10,21 @@
 //  * up to NR = 8 sources advancing together: one multi-RHS Dslash per
 //    half-step (links read once for all NR), one fused update of r and one of
 //    every (x_j, p_j) per iteration, each source with its own device-side
-//    scalars and gate;
-//  * FP32 vectors and links; FP64 solutions x_j, flushed from the FP32
-//    accumulators at every reliable update;
+//    scalars and gate; shifts retire once their zeta-scaled residual meets the
+//    target;
+//  * precision: FP64 vectors and links until every source's |r| is below
+//    QPHIX_MSX_SWITCH (default 1e-3) times |b| -- the shifted systems cannot
+//    be corrected later, and the rounding of the large early steps is what
+//    limits their accuracy -- then FP32 vectors and links, with FP64 solutions
+//    x_j flushed from FP32 accumulators at every reliable update;
 //  * reliable updates on the base system: when |r| has dropped 10x (or looks
 //    converged, or the step budget is spent) that source pauses, its base true
 //    residual b - (M + sigma_low) x_low is recomputed with the FP64 Dslash and
 //    replaces r (residual replacement); it stops once that true residual meets
 //    the target;
 //  * refinement: every shifted solution is then checked in FP64 and, where its
-//    own true residual misses the target, finished by mixed-precision CG from
-//    its multi-shift value.
+//    own true residual misses the target, finished by the same batched solver
+//    on its correction equation, sources of one shift together.
 #include "blas.h"
 #include "dslash_mrhs.h"
 #include "handles.h"
@@ -30,6 +35,7 @@
 #include <map>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace qphix {
@@ -54,9 +60,9 @@ __global__ void __launch_bounds__(FINALIZE_BLOCK)
 // r_r += beta_low,r Ap_r ; reduce |r_r|^2 per source.  One work-item per
 // (site, source), resident grid walking XCD slabs of nlog logical blocks;
 // one partial per block and source.
-template <int NR>
+template <typename T, int NR>
 __global__ void __launch_bounds__(BLAS_BLOCK)
-    k_msx_update_r(float *rs, const float *Ap, int vh, const SolverState *st, double *partials, int nlog) {
+    k_msx_update_r(T *rs, const T *Ap, int vh, const SolverState *st, double *partials, int nlog) {
   const unsigned mask = rhs_mask<NR>(st);
   if (mask == 0) return;
   const XcdWalk wk(nlog);
@@ -67,17 +73,17 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     const int t = blk * BLAS_BLOCK + threadIdx.x;
     const int i = t / NR, r = t % NR;
     if (i >= vh || !((mask >> r) & 1u)) continue;
-    const float bl = (float)st[r].beta_i[st[r].j_low];
-    cplx<float> rv[3], av[3];
-    Spin<float>::load(rs, t, rv);
-    Spin<float>::load(Ap, t, av);
+    const T bl = (T)st[r].beta_i[st[r].j_low];
+    cplx<T> rv[3], av[3];
+    Spin<T>::load(rs, t, rv);
+    Spin<T>::load(Ap, t, av);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       rv[c].re += bl * av[c].re;
       rv[c].im += bl * av[c].im;
       v += (double)rv[c].re * rv[c].re + (double)rv[c].im * rv[c].im;
     }
-    Spin<float>::store(rs, t, rv);
+    Spin<T>::store(rs, t, rv);
   }
   const double sum = lane_block_sum<NR, BLAS_BLOCK>(v);
   if (threadIdx.x < NR) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sum;
@@ -85,24 +91,24 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
 
 // x_j += beta_j p_j ; p_j = zeta_j r + alpha_j p_j  for every shift j < num_now,
 // one work-item per (site, source); layout [shift][site][source][6].
-template <int NR>
+template <typename T, int NR>
 __global__ void __launch_bounds__(BLAS_BLOCK)
-    k_msx_update_pm(float *xs, float *p, const float *rs, int vh, int nshift, const SolverState *st) {
+    k_msx_update_pm(T *xs, T *p, const T *rs, int vh, int nshift, const SolverState *st) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= vh * NR) return;
   const int r = t % NR;
   const SolverState &s = st[r];
   if (!s.active) return;
   const size_t stride = (size_t)vh * NR;
-  cplx<float> rv[3];
-  Spin<float>::load(rs, t, rv);
+  cplx<T> rv[3];
+  Spin<T>::load(rs, t, rv);
   const int nn = s.num_now;
   for (int j = 0; j < nn && j < nshift; ++j) {
     if (s.sdone[j] == 1) continue;
-    const float bj = (float)s.beta_i[j], aj = (float)s.alpha[j], zj = (float)s.zeta_ip1[j];
-    cplx<float> xv[3], pv[3];
-    Spin<float>::load(xs + 6 * stride * j, t, xv);
-    Spin<float>::load(p + 6 * stride * j, t, pv);
+    const T bj = (T)s.beta_i[j], aj = (T)s.alpha[j], zj = (T)s.zeta_ip1[j];
+    cplx<T> xv[3], pv[3];
+    Spin<T>::load(xs + 6 * stride * j, t, xv);
+    Spin<T>::load(p + 6 * stride * j, t, pv);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       xv[c].re += bj * pv[c].re;
@@ -110,8 +116,8 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
       pv[c].re = zj * rv[c].re + aj * pv[c].re;
       pv[c].im = zj * rv[c].im + aj * pv[c].im;
     }
-    Spin<float>::store(xs + 6 * stride * j, t, xv);
-    Spin<float>::store(p + 6 * stride * j, t, pv);
+    Spin<T>::store(xs + 6 * stride * j, t, xv);
+    Spin<T>::store(p + 6 * stride * j, t, pv);
   }
 }
 
@@ -125,34 +131,36 @@ __global__ void k_msx_commit(SolverState *st, int nr) {
   if (st[r].ru) st[r].active = 0;
 }
 
-// Source slot r: r_s = p_j = (float) b for all j, x_s = 0.
+// Source slot r: r_s = p_j = (T) b for all j, x_s = 0.
+template <typename T>
 __global__ void __launch_bounds__(BLAS_BLOCK)
-    k_msx_init(float *rs, float *p, float *xs, const double *b, int vh, int NR, int r, int nshift) {
+    k_msx_init(T *rs, T *p, T *xs, const double *b, int vh, int NR, int r, int nshift) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   const size_t stride = (size_t)vh * NR;
   for (int e = 0; e < 6; ++e) {
-    const float v = (float)b[6 * (size_t)i + e];
+    const T v = (T)b[6 * (size_t)i + e];
     rs[6 * ((size_t)i * NR + r) + e] = v;
     for (int j = 0; j < nshift; ++j) {
       p[6 * (stride * j + (size_t)i * NR + r) + e] = v;
-      xs[6 * (stride * j + (size_t)i * NR + r) + e] = 0.f;
+      xs[6 * (stride * j + (size_t)i * NR + r) + e] = (T)0;
     }
   }
 }
 
 // x64_j += x_s[j] (slot r); x_s[j] = 0
+template <typename T>
 __global__ void __launch_bounds__(BLAS_BLOCK)
-    k_msx_flush(double *const *x64, float *xs, int vh, int NR, int r, int nshift) {
+    k_msx_flush(double *const *x64, T *xs, int vh, int NR, int r, int nshift) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   const size_t stride = (size_t)vh * NR;
   for (int j = 0; j < nshift; ++j) {
-    float *s = xs + 6 * (stride * j + (size_t)i * NR + r);
+    T *s = xs + 6 * (stride * j + (size_t)i * NR + r);
     double *d = x64[j] + 6 * (size_t)i;
     for (int e = 0; e < 6; ++e) {
       d[e] += (double)s[e];
-      s[e] = 0.f;
+      s[e] = (T)0;
     }
   }
 }
@@ -165,12 +173,20 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   for (int e = 0; e < 6; ++e) rs[6 * ((size_t)i * NR + r) + e] = (float)r64[6 * (size_t)i + e];
 }
 
-// Resume a paused source with the FP64 true residual's norm.
-__global__ void k_msx_resume(SolverState *s, const SolverState *t) {
+// Resume a paused source with the FP64 true residual's norm; delta > 0 also
+// sets the next update's drop factor.
+__global__ void k_msx_resume(SolverState *s, const SolverState *t, double delta) {
   s->rsq = t->rsq;
   s->mx_maxr = sqrt(t->rsq);
   s->ru = 0;
   s->active = 1;
+  if (delta > 0) s->mx_delta = delta;
+}
+
+// FP64 -> FP32 copy (the direction vectors at the end of the FP64 phase).
+__global__ void __launch_bounds__(BLAS_BLOCK) k_msx_demote(float *dst, const double *src, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
 }
 
 // Resident grid of a lane-mapped kernel: CUs x resident blocks per CU (from
@@ -184,52 +200,66 @@ template <typename K> static int resident_grid(K kernel, int nlog) {
   return std::max(N_XCD, std::min(need, g));
 }
 
-template <int NR, bool LANE, int NL, int EPI> static int mrhs_grid_of(int vh) {
-  if (!LANE) return (vh + DSLASH_BLOCK - 1) / DSLASH_BLOCK;
+// Logical 256-item blocks of a multi-RHS launch with K sources per work-item.
+inline int mrhs_nlog(int vh, int NR, int K) {
+  return (int)(((size_t)vh * (NR / K) + DSLASH_BLOCK - 1) / DSLASH_BLOCK);
+}
+
+template <typename T, int NR, int K, int NL, int EPI> static int mrhs_grid_of(int vh) {
   static int g = 0;  // per instantiation; the lattice is fixed between QPHIX_init and QPHIX_finalize
   static int g_vh = -1;
   if (g_vh != vh) {
-    g = resident_grid(k_dslash_mrhs_lane<NL, NR, EPI>, (int)(((size_t)vh * NR + DSLASH_BLOCK - 1) / DSLASH_BLOCK));
+    g = resident_grid(k_dslash_mrhs<T, NL, NR, K, EPI>, mrhs_nlog(vh, NR, K));
     g_vh = vh;
   }
   return g;
 }
 
-// Blocks of a multi-RHS Dslash launch (for EPI_PKP, the partial count per source).
-template <int NR> static int mrhs_blocks(int vh, bool lane, int recon, int epi) {
-  if (!lane) return (vh + DSLASH_BLOCK - 1) / DSLASH_BLOCK;
+template <typename T, int NR, int K> static int mrhs_blocks_k(int vh, int recon, int epi) {
   if (recon == 12)
-    return epi == EPI_PLAIN ? mrhs_grid_of<NR, true, 12, EPI_PLAIN>(vh) : mrhs_grid_of<NR, true, 12, EPI_PKP>(vh);
-  return epi == EPI_PLAIN ? mrhs_grid_of<NR, true, 18, EPI_PLAIN>(vh) : mrhs_grid_of<NR, true, 18, EPI_PKP>(vh);
+    return epi == EPI_PLAIN ? mrhs_grid_of<T, NR, K, 12, EPI_PLAIN>(vh) : mrhs_grid_of<T, NR, K, 12, EPI_PKP>(vh);
+  return epi == EPI_PLAIN ? mrhs_grid_of<T, NR, K, 18, EPI_PLAIN>(vh) : mrhs_grid_of<T, NR, K, 18, EPI_PKP>(vh);
 }
 
-template <int NR, bool LANE, int NL, int EPI> static void mrhs_launch1(const MrhsArgs &a) {
-  const int nb = mrhs_grid_of<NR, LANE, NL, EPI>(a.g.vh);
-  if (LANE) hipLaunchKernelGGL((k_dslash_mrhs_lane<NL, NR, EPI>), dim3(nb), dim3(DSLASH_BLOCK), 0, ctx().stream, a);
-  else hipLaunchKernelGGL((k_dslash_mrhs<NL, NR, EPI>), dim3(nb), dim3(DSLASH_BLOCK), 0, ctx().stream, a);
+// Blocks of a multi-RHS Dslash launch (for EPI_PKP, the partial count per source).
+template <int NR, typename T = float> static int mrhs_blocks(int vh, int K, int recon, int epi) {
+  if (K >= 4 && NR % 4 == 0) return mrhs_blocks_k<T, NR, (NR % 4 == 0 ? 4 : 1)>(vh, recon, epi);
+  if (K >= 2 && NR % 2 == 0) return mrhs_blocks_k<T, NR, (NR % 2 == 0 ? 2 : 1)>(vh, recon, epi);
+  return mrhs_blocks_k<T, NR, 1>(vh, recon, epi);
 }
 
-template <int NR, bool LANE>
-static void mrhs_launch(const LinksT<float> &Lf, int epi, const MrhsArgs &a) {
-  if (Lf.recon == 12) {
-    if (epi == EPI_PLAIN) mrhs_launch1<NR, LANE, 12, EPI_PLAIN>(a);
-    else mrhs_launch1<NR, LANE, 12, EPI_PKP>(a);
+template <typename T, int NR, int K>
+static void mrhs_launch(const LinksT<T> &L, int epi, MrhsArgsT<T> a) {
+  const int vh = a.g.vh;
+  a.nlog = mrhs_nlog(vh, NR, K);
+  if (L.recon == 12) {
+    if (epi == EPI_PLAIN)
+      hipLaunchKernelGGL((k_dslash_mrhs<T, 12, NR, K, EPI_PLAIN>), dim3(mrhs_grid_of<T, NR, K, 12, EPI_PLAIN>(vh)),
+                         dim3(DSLASH_BLOCK), 0, ctx().stream, a);
+    else
+      hipLaunchKernelGGL((k_dslash_mrhs<T, 12, NR, K, EPI_PKP>), dim3(mrhs_grid_of<T, NR, K, 12, EPI_PKP>(vh)),
+                         dim3(DSLASH_BLOCK), 0, ctx().stream, a);
   } else {
-    if (epi == EPI_PLAIN) mrhs_launch1<NR, LANE, 18, EPI_PLAIN>(a);
-    else mrhs_launch1<NR, LANE, 18, EPI_PKP>(a);
+    if (epi == EPI_PLAIN)
+      hipLaunchKernelGGL((k_dslash_mrhs<T, 18, NR, K, EPI_PLAIN>), dim3(mrhs_grid_of<T, NR, K, 18, EPI_PLAIN>(vh)),
+                         dim3(DSLASH_BLOCK), 0, ctx().stream, a);
+    else
+      hipLaunchKernelGGL((k_dslash_mrhs<T, 18, NR, K, EPI_PKP>), dim3(mrhs_grid_of<T, NR, K, 18, EPI_PKP>(vh)),
+                         dim3(DSLASH_BLOCK), 0, ctx().stream, a);
   }
 }
 
-// Multi-RHS Dslash on the [vh][NR][6] interleaved layout.  lane selects the
-// thread mapping; EPI_PKP leaves NR x mrhs_blocks<NR>(...) partial sums.
-template <int NR>
-static void mrhs_dslash(const LinksT<float> &Lf, int cb, float *out, const float *in, int epi,
-                        float s, const float *aux, const SolverState *st, bool lane) {
+// Multi-RHS Dslash on the [vh][NR][6] interleaved layout, K sources per
+// work-item (1, 2 or 4; reduced to a divisor of NR); EPI_PKP leaves
+// NR x mrhs_blocks<NR, T>(vh, K, ...) partial sums.
+template <int NR, typename T>
+static void mrhs_dslash(const LinksT<T> &L, int cb, T *out, const T *in, int epi, T s, const T *aux,
+                        const SolverState *st, int K) {
   const Geometry &gg = ctx().g;
-  MrhsArgs a{};
-  a.fat = reinterpret_cast<const float4 *>(Lf.fat[cb]);
-  a.lng = reinterpret_cast<const float4 *>(Lf.lng[cb]);
-  a.lsign = Lf.lsign[cb];
+  MrhsArgsT<T> a{};
+  a.fat = reinterpret_cast<const typename vec16<T>::type *>(L.fat[cb]);
+  a.lng = reinterpret_cast<const typename vec16<T>::type *>(L.lng[cb]);
+  a.lsign = L.lsign[cb];
   a.in = in;
   a.out = out;
   a.aux = aux;
@@ -238,9 +268,9 @@ static void mrhs_dslash(const LinksT<float> &Lf, int cb, float *out, const float
   a.s = s;
   a.partials = ctx().partials;
   a.states = st;
-  a.nlog = (int)(((size_t)gg.vh * NR + DSLASH_BLOCK - 1) / DSLASH_BLOCK);
-  if (lane) mrhs_launch<NR, true>(Lf, epi, a);
-  else mrhs_launch<NR, false>(Lf, epi, a);
+  if (K >= 4 && NR % 4 == 0) mrhs_launch<T, NR, (NR % 4 == 0 ? 4 : 1)>(L, epi, a);
+  else if (K >= 2 && NR % 2 == 0) mrhs_launch<T, NR, (NR % 2 == 0 ? 2 : 1)>(L, epi, a);
+  else mrhs_launch<T, NR, 1>(L, epi, a);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -269,7 +299,7 @@ __global__ void k_mrhs_unpack(float *const *dst, const float *src, int vh, int n
 
 template <int NR>
 static double dslash_multi(LinksT<float> *L, QPHIX_F3_ColorVector **out, QPHIX_F3_ColorVector **in,
-                           int n, int cb, int reps, bool lane) {
+                           int n, int cb, int reps, int K) {
   const int vh = ctx().g.vh;
   const size_t nvr = 6 * (size_t)vh * NR;
   char *w = (char *)ws2_get(2 * nvr * sizeof(float) + NR * sizeof(SolverState) + 2 * NR * sizeof(float *) + 1024);
@@ -295,7 +325,7 @@ static double dslash_multi(LinksT<float> *L, QPHIX_F3_ColorVector **out, QPHIX_F
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, ctx().stream));
-    for (int k = 0; k < reps; ++k) mrhs_dslash<NR>(*L, cb, pout, pin, EPI_PLAIN, 0.f, nullptr, st, lane);
+    for (int k = 0; k < reps; ++k) mrhs_dslash<NR, float>(*L, cb, pout, pin, EPI_PLAIN, 0.f, nullptr, st, K);
     HIP_CHECK(hipEventRecord(e1, ctx().stream));
     HIP_CHECK(hipEventSynchronize(e1));
     float f = 0;
@@ -304,7 +334,7 @@ static double dslash_multi(LinksT<float> *L, QPHIX_F3_ColorVector **out, QPHIX_F
     HIP_CHECK(hipEventDestroy(e0));
     HIP_CHECK(hipEventDestroy(e1));
   } else {
-    mrhs_dslash<NR>(*L, cb, pout, pin, EPI_PLAIN, 0.f, nullptr, st, lane);
+    mrhs_dslash<NR, float>(*L, cb, pout, pin, EPI_PLAIN, 0.f, nullptr, st, K);
   }
   hipLaunchKernelGGL(k_mrhs_unpack, dim3(nbt), dim3(BLAS_BLOCK), 0, ctx().stream, ptrs + NR, pout, vh, n, NR);
   HIP_CHECK(hipGetLastError());
@@ -325,11 +355,11 @@ static double dslash_multi_api(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVec
   const MrhsTune &tune = autotune_mrhs(*L);
   if (tune.nr == 0) return -1.0;
   const int nr = n == 1 ? 1 : n == 2 ? 2 : n <= 4 ? 4 : 8;
-  const bool lane = tune.lane[nr];
-  if (nr == 1) return dslash_multi<1>(L, out, in, n, cb, reps, lane);
-  if (nr == 2) return dslash_multi<2>(L, out, in, n, cb, reps, lane);
-  if (nr == 4) return dslash_multi<4>(L, out, in, n, cb, reps, lane);
-  return dslash_multi<8>(L, out, in, n, cb, reps, lane);
+  const int K = tune.k[nr];
+  if (nr == 1) return dslash_multi<1>(L, out, in, n, cb, reps, K);
+  if (nr == 2) return dslash_multi<2>(L, out, in, n, cb, reps, K);
+  if (nr == 4) return dslash_multi<4>(L, out, in, n, cb, reps, K);
+  return dslash_multi<8>(L, out, in, n, cb, reps, K);
 }
 
 } // namespace qphix
@@ -343,16 +373,17 @@ void QPHIX_F3_asqtad_dslash_multi(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_Color
   dslash_multi_api(L, out, in, nvec, parity, 0);
 }
 
-int QPHIX_F3_asqtad_mrhs_tune(QPHIX_F3_FermionLinksAsqtad *L, double *us) {
+int QPHIX_F3_asqtad_mrhs_tune(QPHIX_F3_FermionLinksAsqtad *L, double *us, int *best_k) {
   QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called: asqtad_mrhs_tune", 0);
   QPHIX_REQUIRE(L && L->fat[0] && L->fat[1], "asqtad_mrhs_tune: both link parities are required", 0);
   const MrhsTune &t = autotune_mrhs(*L);
   if (us) {
     const int nrs[4] = {1, 2, 4, 8};
     for (int c = 0; c < 4; ++c)
-      for (int lane = 0; lane < 2; ++lane) us[2 * c + lane] = t.us[nrs[c]][lane];
+      for (int q = 0; q < 3; ++q) us[3 * c + q] = t.us[nrs[c]][q];
   }
-  return t.nr == 0 ? 0 : t.nr * 2 + (t.lane[t.nr] ? 1 : 0);
+  if (best_k) *best_k = t.nr ? t.k[t.nr] : 0;
+  return t.nr;
 }
 
 double QPHIX_F3_asqtad_dslash_multi_time(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVector *out[],
@@ -392,12 +423,12 @@ int QPHIX_D3_asqtad_invert_multi_rhs(QPHIX_info_t *info, QPHIX_D3_FermionLinksAs
   if (tune.nr == 0) return 0;                 // workspace allocation failed (error recorded)
   int nb = rhs_block > 0 ? rhs_block : tune.nr;
   while (nb > 1 && nb / 2 >= nrhs) nb /= 2;   // no block wider than the sources need
-  const bool lane = tune.lane[nb];
+  const int K = tune.k[nb];
   switch (nb) {
-  case 1: return invert_multi_rhs<1>(info, L, inv, res, mass, nshift, nrhs, out, in, lane);
-  case 2: return invert_multi_rhs<2>(info, L, inv, res, mass, nshift, nrhs, out, in, lane);
-  case 4: return invert_multi_rhs<4>(info, L, inv, res, mass, nshift, nrhs, out, in, lane);
-  default: return invert_multi_rhs<8>(info, L, inv, res, mass, nshift, nrhs, out, in, lane);
+  case 1: return invert_multi_rhs<1>(info, L, inv, res, mass, nshift, nrhs, out, in, K);
+  case 2: return invert_multi_rhs<2>(info, L, inv, res, mass, nshift, nrhs, out, in, K);
+  case 4: return invert_multi_rhs<4>(info, L, inv, res, mass, nshift, nrhs, out, in, K);
+  default: return invert_multi_rhs<8>(info, L, inv, res, mass, nshift, nrhs, out, in, K);
   }
 }
 

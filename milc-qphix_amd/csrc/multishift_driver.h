@@ -9,18 +9,53 @@ struct BatchStats {
   int shift_iters[8][MAXNDMASS];        // step at which each shift converged
 };
 
+// One batch of multi-shift steps for NR sources in precision T: kernels only
+// (no memsets or copies), every argument fixed for the chunk and phase, every
+// scalar read from the device states -- so it can be captured as a hipGraph
+// and replayed; a step whose source is paused or finished is a no-op through
+// its gate.
+template <int NR, typename T>
+static void msx_batch(const LinksT<T> &Lk, int parity, int batch, T *p, T *xs, T *rs, T *Ap, T *tmp,
+                      int j_low, double offset_low, SolverState *st, int K, int nbd, int nbr,
+                      int nlog, int nshift) {
+  const int vh = ctx().g.vh;
+  const size_t nvr = 6 * (size_t)vh * NR;
+  const T *plow = p + (size_t)j_low * nvr;
+  for (int k = 0; k < batch; ++k) {
+    mrhs_dslash<NR, T>(Lk, 1 - parity, tmp, plow, EPI_PLAIN, (T)0, nullptr, st, K);
+    mrhs_dslash<NR, T>(Lk, parity, Ap, tmp, EPI_PKP, (T)-offset_low, plow, st, K);
+    hipLaunchKernelGGL(k_finalize_multi<NR>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream,
+                       ctx().partials, nbd, st, (int)SOP_MS_PKP);
+    hipLaunchKernelGGL((k_msx_update_r<T, NR>), dim3(nbr), dim3(BLAS_BLOCK), 0, ctx().stream, rs, Ap, vh, st,
+                       ctx().partials, nlog);
+    hipLaunchKernelGGL(k_finalize_multi<NR>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream,
+                       ctx().partials, nbr, st, (int)SOP_MSX_RSQ2);
+    hipLaunchKernelGGL((k_msx_update_pm<T, NR>), dim3((vh * NR + BLAS_BLOCK - 1) / BLAS_BLOCK), dim3(BLAS_BLOCK),
+                       0, ctx().stream, xs, p, rs, vh, nshift, st);
+    hipLaunchKernelGGL(k_msx_commit, dim3(1), dim3(NR), 0, ctx().stream, st, NR);
+  }
+}
+
 // One chunk of up to NR sources on one parity.  x64[r*nshift+j] are the FP64
 // solution halves (zeroed here), b64[r] the source halves, resid_src[r] each
 // source's squared relative target.
+//
+// switch_frac > 0: the chunk first iterates in FP64 (FP64 multi-RHS Dslash and
+// vectors) until every source's |r| has fallen below switch_frac * |b|, then
+// in FP32 with FP64 reliable updates.  The shifted systems have no residual of
+// their own to correct, so their solutions keep the rounding of every step;
+// steps taken while |r| ~ |b| dominate it, and running those in FP64 leaves
+// the shifted solutions at ~eps32 * switch_frac instead of ~eps32.
+// switch_frac = 0: FP32 from the start.
 template <int NR>
 static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const double *mass,
                       const double *resid_src, int max_cg, double *const *x64,
-                      const double *const *b64, bool lane, BatchStats &bs) {
+                      const double *const *b64, int K, double switch_frac, BatchStats &bs) {
   const int vh = ctx().g.vh;
   const size_t nv = 6 * (size_t)vh, nvr = nv * NR;
   const LinksT<float> &Lf = *L->sloppy;
   const int other = 1 - parity;
-  // workspace: p, xs [nshift][vh][NR][6]; rs, Ap, tmp [vh][NR][6]; r64, tmp64; states; pointers
+  // FP32 workspace: p, xs [nshift][vh][NR][6]; rs, Ap, tmp [vh][NR][6]; r64, tmp64; states; pointers
   char *w = (char *)ws2_get((2 * (size_t)nshift + 3) * nvr * sizeof(float) + 2 * nv * sizeof(double) +
                             (NR + 2) * sizeof(SolverState) + NR * nshift * sizeof(double *) + 4096);
   if (!w) return false;
@@ -29,11 +64,25 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
   SolverState *st = (SolverState *)(((uintptr_t)(tmp64 + nv) + 255) & ~(uintptr_t)255);
   SolverState *tst = st + NR;
   double **dx = (double **)(tst + 1);
+  // FP64-phase workspace, same shapes
+  bool hi = switch_frac > 0;
+  double *w64 = nullptr;
+  if (hi) {
+    w64 = (double *)dev_alloc((2 * (size_t)nshift + 3) * nvr * sizeof(double));
+    if (!w64) return false;
+  }
+  double *p_d = w64, *xs_d = hi ? p_d + nshift * nvr : nullptr;
+  double *rs_d = hi ? xs_d + nshift * nvr : nullptr, *Ap_d = hi ? rs_d + nvr : nullptr;
+  double *tmp_d = hi ? Ap_d + nvr : nullptr;
   const int nb = (vh + BLAS_BLOCK - 1) / BLAS_BLOCK;
-  const int nbd = mrhs_blocks<NR>(vh, lane, Lf.recon, EPI_PKP);
   const int nlog = (int)(((size_t)vh * NR + BLAS_BLOCK - 1) / BLAS_BLOCK);
-  const int nbr = resident_grid(k_msx_update_r<NR>, nlog);
-  if ((size_t)NR * std::max(nbd, nbr) > ctx().partials_len) {
+  const int K64 = std::min(K, 2);  // FP64: twice the registers per source
+  const int nbd = mrhs_blocks<NR>(vh, K, Lf.recon, EPI_PKP);
+  const int nbr = resident_grid(k_msx_update_r<float, NR>, nlog);
+  const int nbd64 = mrhs_blocks<NR, double>(vh, K64, L->recon, EPI_PKP);
+  const int nbr64 = resident_grid(k_msx_update_r<double, NR>, nlog);
+  if ((size_t)NR * std::max(std::max(nbd, nbr), std::max(nbd64, nbr64)) > ctx().partials_len) {
+    dev_free(w64);
     fail(__FILE__, __LINE__, "multi-RHS solver: reduction scratch too small for this block size");
     return false;
   }
@@ -85,10 +134,15 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     s.rsq = t.src_norm;
     s.rsqstop = resid_src[r] * t.src_norm;
     s.mx_maxr = std::sqrt(t.src_norm);
-    s.mx_delta = delta;
+    s.mx_delta = hi ? switch_frac : delta;
     s.max_cg = max_cg;
     s.active = 1;
-    hipLaunchKernelGGL(k_msx_init, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, xs, b64[r], vh, NR, r, nshift);
+    if (hi)
+      hipLaunchKernelGGL(k_msx_init<double>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs_d, p_d, xs_d,
+                         b64[r], vh, NR, r, nshift);
+    else
+      hipLaunchKernelGGL(k_msx_init<float>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, xs, b64[r],
+                         vh, NR, r, nshift);
   }
   // states and pointer table staged at distinct offsets of the pinned buffer
   // (both copies are in flight together)
@@ -100,37 +154,71 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
   HIP_CHECK(hipMemcpyAsync(dx, pin + ptr_off, (size_t)NR * nshift * sizeof(double *), hipMemcpyHostToDevice, ctx().stream));
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
 
-  const float *plow = p + (size_t)j_low * nvr;
   const int batch = 16;
-  // One batch of steps: kernels only (no memsets or copies), every argument
-  // fixed for this chunk, every scalar read from the device states -- so it is
-  // captured once as a hipGraph and replayed; a step whose source is paused or
-  // finished is a no-op through its gate.
   auto enqueue_batch = [&]() {
-    for (int k = 0; k < batch; ++k) {
-      mrhs_dslash<NR>(Lf, other, tmp, plow, EPI_PLAIN, 0.f, nullptr, st, lane);
-      mrhs_dslash<NR>(Lf, parity, Ap, tmp, EPI_PKP, (float)-offset_low, plow, st, lane);
-      hipLaunchKernelGGL(k_finalize_multi<NR>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream,
-                         ctx().partials, nbd, st, (int)SOP_MS_PKP);
-      hipLaunchKernelGGL(k_msx_update_r<NR>, dim3(nbr), dim3(BLAS_BLOCK), 0, ctx().stream, rs, Ap, vh, st,
-                         ctx().partials, nlog);
-      hipLaunchKernelGGL(k_finalize_multi<NR>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream,
-                         ctx().partials, nbr, st, (int)SOP_MSX_RSQ2);
-      hipLaunchKernelGGL(k_msx_update_pm<NR>, dim3((vh * NR + BLAS_BLOCK - 1) / BLAS_BLOCK), dim3(BLAS_BLOCK), 0,
-                         ctx().stream, xs, p, rs, vh, nshift, st);
-      hipLaunchKernelGGL(k_msx_commit, dim3(1), dim3(NR), 0, ctx().stream, st, NR);
-    }
+    if (hi)
+      msx_batch<NR, double>(*L, parity, batch, p_d, xs_d, rs_d, Ap_d, tmp_d, j_low, offset_low, st, K64,
+                            nbd64, nbr64, nlog, nshift);
+    else
+      msx_batch<NR, float>(Lf, parity, batch, p, xs, rs, Ap, tmp, j_low, offset_low, st, K, nbd, nbr,
+                           nlog, nshift);
   };
   hipGraphExec_t gexec = nullptr;
-  if (ctx().graphs) {
-    (void)mrhs_blocks<NR>(vh, lane, Lf.recon, EPI_PLAIN);  // grid queries happen before capture
+  auto capture = [&]() {
+    if (gexec) HIP_CHECK(hipGraphExecDestroy(gexec));
+    gexec = nullptr;
+    if (!ctx().graphs) return;
+    // grid queries happen before capture
+    (void)mrhs_blocks<NR>(vh, K, Lf.recon, EPI_PLAIN);
+    (void)mrhs_blocks<NR, double>(vh, K64, L->recon, EPI_PLAIN);
     hipGraph_t graph = nullptr;
     HIP_CHECK(hipStreamBeginCapture(ctx().stream, hipStreamCaptureModeThreadLocal));
     enqueue_batch();
     HIP_CHECK(hipStreamEndCapture(ctx().stream, &graph));
     HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphDestroy(graph));
-  }
+  };
+  capture();
+
+  // FP64 true residual b - (M + sigma_low) x_low of source r into r64; returns
+  // its SolverState (rsq, final_rsq = rsq / |b|^2).
+  auto true_residual = [&](int r) {
+    SolverState t{};
+    t.src_norm = hs[r].src_norm;
+    std::memcpy(ctx().pinned, &t, sizeof(t));
+    HIP_CHECK(hipMemcpyAsync(tst, ctx().pinned, sizeof(t), hipMemcpyHostToDevice, ctx().stream));
+    double *xl = x64[r * nshift + j_low];
+    DslashExtra e1;
+    launch_dslash<double>(*L, other, tmp64, xl, EPI_PLAIN, e1);
+    DslashExtra e2;
+    e2.s = -offset_low;
+    e2.aux = xl;
+    e2.b = b64[r];
+    e2.r = r64;
+    e2.p = r64;
+    e2.state = tst;
+    e2.scalar_op = SOP_MX_TRUE;
+    launch_dslash<double>(*L, parity, r64, tmp64, EPI_RESTART, e2);
+    HIP_CHECK(hipMemcpyAsync(ctx().pinned, tst, sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
+    HIP_CHECK(hipStreamSynchronize(ctx().stream));
+    std::memcpy(&t, ctx().pinned, sizeof(t));
+    return t;
+  };
+  // Source r after a true-residual check: finished, or resumed in FP32.
+  auto settle = [&](int r, const SolverState &t, double new_delta) {
+    bs.ms_iters[r] = hs[r].iteration;
+    bs.final_rsq[r] = t.final_rsq;
+    if (t.final_rsq < resid_src[r] || hs[r].iteration >= max_cg || !std::isfinite(t.rsq)) {
+      done[r] = 1;
+      for (int j = 0; j < nshift; ++j)
+        bs.shift_iters[r][j] = hs[r].sdone[j] ? hs[r].sdone_at[j] : hs[r].iteration;
+      return;
+    }
+    ++bs.updates[r];
+    hipLaunchKernelGGL(k_msx_set_r, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, r64, vh, NR, r);
+    hipLaunchKernelGGL(k_msx_resume, dim3(1), dim3(1), 0, ctx().stream, st + r, tst, new_delta);
+  };
+
   while (true) {
     bool any = false;
     for (int r = 0; r < nrhs; ++r) any |= !done[r];
@@ -141,56 +229,55 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     HIP_CHECK(hipMemcpyAsync(ctx().pinned, st, NR * sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
     HIP_CHECK(hipStreamSynchronize(ctx().stream));
     std::memcpy(hs.data(), ctx().pinned, NR * sizeof(SolverState));
+    if (hi) {
+      // FP64 phase: a source pauses once |r| < switch_frac |b| (or on
+      // convergence or budget); the chunk drops to FP32 when all have.
+      bool all = true;
+      for (int r = 0; r < nrhs; ++r)
+        if (!done[r] && hs[r].active) all = false;
+      if (!all) continue;
+      for (int r = 0; r < nrhs; ++r) {
+        if (done[r]) continue;
+        hipLaunchKernelGGL(k_msx_flush<double>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, dx + r * nshift,
+                           xs_d, vh, NR, r, nshift);
+        settle(r, true_residual(r), delta);
+      }
+      const size_t np = (size_t)nshift * nvr;
+      hipLaunchKernelGGL(k_msx_demote, dim3((unsigned)((np + BLAS_BLOCK - 1) / BLAS_BLOCK)), dim3(BLAS_BLOCK), 0,
+                         ctx().stream, p, p_d, np);
+      HIP_CHECK(hipMemsetAsync(xs, 0, np * sizeof(float), ctx().stream));
+      HIP_CHECK(hipStreamSynchronize(ctx().stream));
+      dev_free(w64);
+      w64 = nullptr;
+      hi = false;
+      capture();
+      continue;
+    }
     for (int r = 0; r < nrhs; ++r) {
       if (done[r] || hs[r].active) continue;
       // reliable update of source r: flush x, FP64 true residual of the base system
-      hipLaunchKernelGGL(k_msx_flush, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, dx + r * nshift, xs, vh, NR, r, nshift);
-      SolverState t{};
-      t.src_norm = hs[r].src_norm;
-      std::memcpy(ctx().pinned, &t, sizeof(t));
-      HIP_CHECK(hipMemcpyAsync(tst, ctx().pinned, sizeof(t), hipMemcpyHostToDevice, ctx().stream));
-      double *xl = x64[r * nshift + j_low];
-      DslashExtra e1;
-      launch_dslash<double>(*L, other, tmp64, xl, EPI_PLAIN, e1);
-      DslashExtra e2;
-      e2.s = -offset_low;
-      e2.aux = xl;
-      e2.b = b64[r];
-      e2.r = r64;
-      e2.p = r64;
-      e2.state = tst;
-      e2.scalar_op = SOP_MX_TRUE;
-      launch_dslash<double>(*L, parity, r64, tmp64, EPI_RESTART, e2);
-      HIP_CHECK(hipMemcpyAsync(ctx().pinned, tst, sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
-      HIP_CHECK(hipStreamSynchronize(ctx().stream));
-      std::memcpy(&t, ctx().pinned, sizeof(t));
-      bs.ms_iters[r] = hs[r].iteration;
-      bs.final_rsq[r] = t.final_rsq;
-      if (t.final_rsq < resid_src[r] || hs[r].iteration >= max_cg || !std::isfinite(t.rsq)) {
-        done[r] = 1;
-        for (int j = 0; j < nshift; ++j)
-          bs.shift_iters[r][j] = hs[r].sdone[j] ? hs[r].sdone_at[j] : hs[r].iteration;
-        continue;
-      }
-      ++bs.updates[r];
-      hipLaunchKernelGGL(k_msx_set_r, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, r64, vh, NR, r);
-      hipLaunchKernelGGL(k_msx_resume, dim3(1), dim3(1), 0, ctx().stream, st + r, tst);
+      hipLaunchKernelGGL(k_msx_flush<float>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, dx + r * nshift, xs,
+                         vh, NR, r, nshift);
+      settle(r, true_residual(r), 0.0);
     }
   }
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
   if (gexec) HIP_CHECK(hipGraphExecDestroy(gexec));
+  dev_free(w64);
   return true;
 }
 
-// Autotuner: per RHS block size (1, 2, 4, 8) the faster of the two thread
-// mappings, and the block size with the lowest time per source, measured once
-// per (lattice, Naik storage) with a plain multi-RHS Dslash and cached.  The
-// best size is set by how much of the gathered neighbour data stays in L2.
-// QPHIX_MRHS_LANE=0/1 pins the mapping.
+// Autotuner: the multi-RHS Dslash is timed for every RHS block size NR in
+// {1, 2, 4, 8} and sources-per-work-item K in {1, 2, 4} (K <= NR), once per
+// (lattice, Naik storage), and cached; the block size with the lowest time per
+// source and its best K drive the batched solver and the multi-vector Dslash.
+// How much of the gathered neighbour data stays in L2 favours NR; the L1
+// traffic of the links (once per K sources) against registers picks K.
+// QPHIX_MRHS_K pins K.
 struct MrhsTune {
   int nr = 0;            // best block size; 0 = tuning failed
-  bool lane[9] = {};     // mapping per block size
-  double us[9][2] = {};  // measured microseconds per source [nr][lane]
+  int k[9] = {};         // best K per block size
+  double us[9][3] = {};  // microseconds per source [nr][K = 1, 2, 4]
 };
 
 static const MrhsTune &autotune_mrhs(const LinksT<float> &Lf) {
@@ -210,39 +297,45 @@ static const MrhsTune &autotune_mrhs(const LinksT<float> &Lf) {
   for (auto &x : hs) x.active = 1;
   std::memcpy(ctx().pinned, hs.data(), 8 * sizeof(SolverState));
   HIP_CHECK(hipMemcpyAsync(st, ctx().pinned, 8 * sizeof(SolverState), hipMemcpyHostToDevice, ctx().stream));
-  const char *pin_lane = std::getenv("QPHIX_MRHS_LANE");
+  const char *pin_k = std::getenv("QPHIX_MRHS_K");
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
   MrhsTune t;
   double best_t = 1e300;
   const int cand[4] = {1, 2, 4, 8};
+  const int kc[3] = {1, 2, 4};
   for (int c = 0; c < 4; ++c) {
     const int nr = cand[c];
     double best_m = 1e300;
-    for (int lane = 0; lane < 2; ++lane) {
-      if (pin_lane && std::atoi(pin_lane) != lane) continue;
+    for (int q = 0; q < 3; ++q) {
+      const int K = kc[q];
+      if (K > nr || (pin_k && std::atoi(pin_k) != K)) continue;
       auto launch = [&]() {
-        if (nr == 1) mrhs_dslash<1>(Lf, 0, out, in, EPI_PLAIN, 0.f, nullptr, st, lane);
-        else if (nr == 2) mrhs_dslash<2>(Lf, 0, out, in, EPI_PLAIN, 0.f, nullptr, st, lane);
-        else if (nr == 4) mrhs_dslash<4>(Lf, 0, out, in, EPI_PLAIN, 0.f, nullptr, st, lane);
-        else mrhs_dslash<8>(Lf, 0, out, in, EPI_PLAIN, 0.f, nullptr, st, lane);
+        if (nr == 1) mrhs_dslash<1, float>(Lf, 0, out, in, EPI_PLAIN, 0.f, nullptr, st, K);
+        else if (nr == 2) mrhs_dslash<2, float>(Lf, 0, out, in, EPI_PLAIN, 0.f, nullptr, st, K);
+        else if (nr == 4) mrhs_dslash<4, float>(Lf, 0, out, in, EPI_PLAIN, 0.f, nullptr, st, K);
+        else mrhs_dslash<8, float>(Lf, 0, out, in, EPI_PLAIN, 0.f, nullptr, st, K);
       };
       launch();
       HIP_CHECK(hipEventRecord(e0, ctx().stream));
-      for (int k = 0; k < 3; ++k) launch();
+      for (int r = 0; r < 3; ++r) launch();
       HIP_CHECK(hipEventRecord(e1, ctx().stream));
       HIP_CHECK(hipEventSynchronize(e1));
       float ms = 0;
       HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
       const double per = 1e3 * ms / 3.0 / nr;
-      t.us[nr][lane] = per;
-      if (per < best_m) { best_m = per; t.lane[nr] = lane; }
+      t.us[nr][q] = per;
+      if (per < best_m) { best_m = per; t.k[nr] = K; }
     }
     if (best_m < best_t) { best_t = best_m; t.nr = nr; }
   }
   HIP_CHECK(hipEventDestroy(e0));
   HIP_CHECK(hipEventDestroy(e1));
+  if (t.nr == 0) {  // QPHIX_MRHS_K excluded every candidate
+    t.nr = 1;
+    t.k[1] = 1;
+  }
   return cache[key] = t;
 }
 
@@ -254,7 +347,7 @@ __global__ void __launch_bounds__(BLAS_BLOCK) k_axpy64(double *x, const double *
 template <int NR>
 static int invert_multi_rhs(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
                             QPHIX_resid_arg_t **res, const double *mass, int nshift, int nrhs,
-                            QPHIX_D3_ColorVector **out, QPHIX_D3_ColorVector **in, bool lane) {
+                            QPHIX_D3_ColorVector **out, QPHIX_D3_ColorVector **in, int K) {
   const double t0 = wall_now();
   const int vh = ctx().g.vh;
   const size_t nv = 6 * (size_t)vh;
@@ -279,6 +372,10 @@ static int invert_multi_rhs(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_
   const int nb = (int)((nv + BLAS_BLOCK - 1) / BLAS_BLOCK);
   const char *vb = std::getenv("QPHIX_VERBOSE");
   const bool verbose = vb && std::atoi(vb) > 0;
+  // |r| / |b| at which the multi-shift iteration drops from FP64 to FP32
+  // (QPHIX_MSX_SWITCH; 0 = FP32 throughout)
+  const char *sw = std::getenv("QPHIX_MSX_SWITCH");
+  const double switch_frac = sw ? std::atof(sw) : 1e-3;
   for (int pass = 0; pass < ntodo; ++pass) {
     const int parity = todo[pass], other = 1 - parity;
     // 1. multi-shift on blocks of NR sources
@@ -292,7 +389,7 @@ static int invert_multi_rhs(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_
         for (int j = 0; j < nshift; ++j) x[r * nshift + j] = out[(r0 + r) * nshift + j]->half[parity];
       }
       BatchStats bs;
-      if (!run_chunk<NR>(L, parity, nr, nshift, mass, rs.data(), max_cg, x.data(), b.data(), lane, bs)) {
+      if (!run_chunk<NR>(L, parity, nr, nshift, mass, rs.data(), max_cg, x.data(), b.data(), K, switch_frac, bs)) {
         dev_free(rbuf);
         info->status = QPHIX_MEM_ERROR;
         return 0;
@@ -308,8 +405,8 @@ static int invert_multi_rhs(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_
     }
     const double t_ms = wall_now();
     if (verbose)
-      std::fprintf(stderr, "[qphix] multi_rhs parity %d: multi-shift phase %.3f s (block %d, %s mapping)\n",
-                   parity, t_ms - t0, NR, lane ? "lane" : "site");
+      std::fprintf(stderr, "[qphix] multi_rhs parity %d: multi-shift phase %.3f s (block %d, %d sources per work-item)\n",
+                   parity, t_ms - t0, NR, K);
     // 2. per-shift FP64 check; sources that miss their target are refined
     //    together, NR at a time, by the same solver on the correction equation
     //    (M + sigma_j) e = b - (M + sigma_j) x_j, then x_j += e.
@@ -363,7 +460,7 @@ static int invert_multi_rhs(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_
             rt[q] = target(rr, j) * bnorm[rr] / rnorm[rr];
           }
           BatchStats bs;
-          if (!run_chunk<NR>(L, parity, nr, 1, &mass[j], rt.data(), max_cg, e.data(), src.data(), lane, bs)) {
+          if (!run_chunk<NR>(L, parity, nr, 1, &mass[j], rt.data(), max_cg, e.data(), src.data(), K, 0.0, bs)) {
             dev_free(rbuf);
             info->status = QPHIX_MEM_ERROR;
             return 0;

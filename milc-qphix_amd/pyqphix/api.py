@@ -161,16 +161,17 @@ def dslash_multi(L, outs, inps, parity, reps=0):
 
 
 def mrhs_tune(L):
-    """QPHIX_F3_asqtad_mrhs_tune: (best block size, per-(site, source) mapping?,
-    {(block, mapping): microseconds per source})."""
+    """QPHIX_F3_asqtad_mrhs_tune: (best block size NR, its best sources per
+    work-item K, {(NR, K): microseconds per source})."""
     assert L.prec == "F"
-    us = (C.c_double * 8)()
-    code = L.s.lib.QPHIX_F3_asqtad_mrhs_tune(L.h, us)
-    if code == 0:
+    us = (C.c_double * 12)()
+    k = C.c_int(0)
+    nr = L.s.lib.QPHIX_F3_asqtad_mrhs_tune(L.h, us, C.byref(k))
+    if nr == 0:
         raise RuntimeError("QPHIX_F3_asqtad_mrhs_tune failed")
-    table = {(nr, m): us[2 * c + k] for c, nr in enumerate((1, 2, 4, 8))
-             for k, m in enumerate(("site", "lane")) if us[2 * c + k] > 0}
-    return code // 2, bool(code & 1), table
+    table = {(n, kk): us[3 * c + q] for c, n in enumerate((1, 2, 4, 8))
+             for q, kk in enumerate((1, 2, 4)) if us[3 * c + q] > 0}
+    return nr, k.value, table
 
 
 def dslash_time(L, out, inp, parity, reps):

@@ -135,7 +135,7 @@ class Lib:
              [_dp, C.POINTER(_dp), C.POINTER(_dp), C.c_int, C.c_int])
         _sig(lib, "QPHIX_F3_asqtad_dslash_multi_time", C.c_double,
              [_dp, C.POINTER(_dp), C.POINTER(_dp), C.c_int, C.c_int, C.c_int])
-        _sig(lib, "QPHIX_F3_asqtad_mrhs_tune", C.c_int, [_dp, C.POINTER(C.c_double)])
+        _sig(lib, "QPHIX_F3_asqtad_mrhs_tune", C.c_int, [_dp, C.POINTER(C.c_double), C.POINTER(C.c_int)])
         _sig(lib, "QPHIX_D3_asqtad_invert_multi_rhs", C.c_int,
              [C.POINTER(Info), _dp, C.POINTER(InvertArg), C.POINTER(C.POINTER(ResidArg)),
               C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_int])
