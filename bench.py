@@ -304,6 +304,8 @@ def main():
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched multi-shift measurement (prints its JSON)")
     ap.add_argument("--no-force", action="store_true")
+    ap.add_argument("--only-force", action="store_true",
+                    help="run only the 24^4 gauge- and fermion-force measurement (prints its JSON)")
     ap.add_argument("--combined", action="store_true",
                     help="run only the 48^3x96 multi-shift + fermion-force case (prints its JSON)")
     args = ap.parse_args()
@@ -317,6 +319,9 @@ def main():
 
     if args.only_batch:
         print(json.dumps({"multishift_batched": batch_multishift(A, F)}), flush=True)
+        return 0
+    if args.only_force:
+        print(json.dumps({"forces_24": force_24(A, F)}), flush=True)
         return 0
     if args.combined:
         print(json.dumps({"combined_48x96": combined_48(A, F)}), flush=True)

@@ -40,9 +40,10 @@ int QPHIX_MI355X_device_info(int *n_cu, double *hbm_gib);
  * +-conj(row0 x row1) in its third row, else 18), 18 = always 18.
  * Applies to links created afterwards. */
 void QPHIX_MI355X_set_long_recon(int mode);
-/* on != 0 (default): the batched multi-shift solver captures each batch of
- * iterations once as a hipGraph and replays it; on == 0: plain launches (the
- * same kernels in the same order, hence identical results). */
+/* on != 0 (default): the solvers (CG, multi-shift, mixed and batched) capture
+ * their batch of iterations once per solve as a hipGraph and replay it;
+ * on == 0: plain launches (the same kernels in the same order, hence
+ * identical results). */
 void QPHIX_MI355X_set_graphs(int on);
 /* Storage chosen for L: 12 or 18. */
 int QPHIX_D3_asqtad_links_recon(QPHIX_D3_FermionLinksAsqtad *L);

@@ -174,7 +174,6 @@ struct DslashExtra {
   void *state = nullptr;       // device SolverState updated by the last block
   int scalar_op = 0;           // ScalarOp run by the last block
   const int *gate = nullptr;   // kernel is a no-op when *gate == 0
-  bool stream_out = false;     // EPI_PLAIN: non-temporal output stores (result not reused soon)
 };
 
 template <typename T>
@@ -190,6 +189,37 @@ void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf);
 int mixed_solve(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
                 QPHIX_resid_arg_t *res, double mass, VectorT<double> *out, VectorT<double> *in,
                 int sloppy_bits);
+
+// A batch of solver iterations captured once as a hipGraph and replayed.
+// The enqueue function must launch kernels only (no memsets or copies), with
+// arguments that stay fixed for the graph's lifetime; every scalar it needs
+// lives in device memory.  With ctx().graphs off, run() just enqueues.
+struct BatchGraph {
+  hipGraphExec_t exec = nullptr;
+  BatchGraph() = default;
+  BatchGraph(const BatchGraph &) = delete;
+  BatchGraph &operator=(const BatchGraph &) = delete;
+  template <typename F> void run(F &&enqueue) {
+    if (!ctx().graphs) {
+      enqueue();
+      return;
+    }
+    if (!exec) {
+      hipGraph_t g = nullptr;
+      HIP_CHECK(hipStreamBeginCapture(ctx().stream, hipStreamCaptureModeThreadLocal));
+      enqueue();
+      HIP_CHECK(hipStreamEndCapture(ctx().stream, &g));
+      HIP_CHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+      HIP_CHECK(hipGraphDestroy(g));
+    }
+    HIP_CHECK(hipGraphLaunch(exec, ctx().stream));
+  }
+  void reset() {
+    if (exec) HIP_CHECK(hipGraphExecDestroy(exec));
+    exec = nullptr;
+  }
+  ~BatchGraph() { reset(); }
+};
 
 // Second workspace, for drivers that call other solvers while holding theirs.
 void *ws2_get(size_t bytes);

@@ -76,6 +76,13 @@ void *ws2_get(size_t bytes) {
 static QPHIX_status_t init_common(QPHIX_layout_t *layout) {
   Context &c = g_ctx;
   if (c.initialised) return QPHIX_SUCCESS;  // F and D share one lattice (C14)
+  // QPHIX_GRAPHS=0 starts the process with graph replay off (read once; a
+  // later QPHIX_MI355X_set_graphs call wins)
+  static bool env_read = false;
+  if (!env_read) {
+    env_read = true;
+    if (const char *e = std::getenv("QPHIX_GRAPHS")) c.graphs = std::atoi(e) != 0;
+  }
   if (!layout || layout->latdim != 4 || !layout->latsize) {
     fail(__FILE__, __LINE__, "QPHIX_init: need a 4-d layout");
     return QPHIX_FAIL;
@@ -150,9 +157,11 @@ static QPHIX_status_t finalize_common() {
   HIP_CHECK(hipStreamDestroy(c.stream));
   const bool keep_abort = c.abort_on_error;
   const bool keep_sync = c.sync_calls;
+  const bool keep_graphs = c.graphs;
   c = Context{};
   c.abort_on_error = keep_abort;
   c.sync_calls = keep_sync;
+  c.graphs = keep_graphs;
   return QPHIX_SUCCESS;
 }
 

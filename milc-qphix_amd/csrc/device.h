@@ -125,7 +125,17 @@ template <> struct Spin<HalfSite> {
     *reinterpret_cast<uint4 *>(p + i) = raw;
   }
   static __device__ __forceinline__ void store_nt(HalfSite *p, int i, const cplx<float> *v) {
-    store(p, i, v);
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    float m = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v[c].re), fabsf(v[c].im)));
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+    __half2 h0 = __floats2half2_rn(v[0].re * inv, v[0].im * inv);
+    __half2 h1 = __floats2half2_rn(v[1].re * inv, v[1].im * inv);
+    __half2 h2 = __floats2half2_rn(v[2].re * inv, v[2].im * inv);
+    const u4 raw = {*reinterpret_cast<unsigned *>(&h0), *reinterpret_cast<unsigned *>(&h1),
+                    *reinterpret_cast<unsigned *>(&h2), __float_as_uint(m)};
+    __builtin_nontemporal_store(raw, reinterpret_cast<u4 *>(p + i));
   }
 };
 

@@ -21,35 +21,31 @@
 
 namespace qphix {
 
-// Kernel variants (see dslash_kernel.h).  Measured on MI355X at 32^4 (Naik
-// 12): XCD slabs + streaming link loads take FP64 from 230 to 207 us and FP32
-// from 104 to 93 us.  Streaming output stores add a few per cent, but they
-// evict the result a solver reads next, so only the stand-alone Dslash (whose
-// output goes back to the caller) uses them.
-constexpr int DSLASH_OPT = OPT_XCD | OPT_NTL;
-constexpr int DSLASH_OPT_STREAM_OUT = OPT_XCD | OPT_NTL | OPT_NTS;
+// Kernel variant of every launch (see dslash_kernel.h).  Measured on MI355X
+// at 32^4 (Naik 12, tile-major links): XCD slabs + streaming link loads take
+// FP64 from 230 to 201 us and FP32 from 104 to 95 us; streaming output stores
+// take them to 188 and 93 us.  The output stores stream even where the next
+// kernel reads the result (a solver's tmp = D p): that pair of Dslashes is 3%
+// faster than with cached stores (FP32 190 vs 196 us, FP64 390 vs 403 us).
+constexpr int DSLASH_OPT = OPT_XCD | OPT_NTL | OPT_NTS;
 
 template <typename TL, int NL, typename SI, typename SO, int EPI>
-static void launch_one(const DslashKArgs<TL, SI, SO> &a, int nblocks, bool stream_out) {
-  if (EPI == EPI_PLAIN && stream_out)
-    hipLaunchKernelGGL((k_dslash<TL, NL, SI, SO, EPI, DSLASH_OPT_STREAM_OUT>), dim3(nblocks),
-                       dim3(DSLASH_BLOCK), 0, ctx().stream, a);
-  else
-    hipLaunchKernelGGL((k_dslash<TL, NL, SI, SO, EPI, DSLASH_OPT>), dim3(nblocks),
-                       dim3(DSLASH_BLOCK), 0, ctx().stream, a);
+static void launch_one(const DslashKArgs<TL, SI, SO> &a, int nblocks) {
+  hipLaunchKernelGGL((k_dslash<TL, NL, SI, SO, EPI, DSLASH_OPT>), dim3(nblocks), dim3(DSLASH_BLOCK), 0,
+                     ctx().stream, a);
 }
 
 template <typename TL, typename SI, typename SO>
-static void dispatch(const DslashKArgs<TL, SI, SO> &a, int recon, int epi, bool so = false) {
+static void dispatch(const DslashKArgs<TL, SI, SO> &a, int recon, int epi) {
   const int nb = (a.g.vh + DSLASH_BLOCK - 1) / DSLASH_BLOCK;
   if (recon == 12) {
-    if (epi == EPI_PLAIN) launch_one<TL, 12, SI, SO, EPI_PLAIN>(a, nb, so);
-    else if (epi == EPI_PKP) launch_one<TL, 12, SI, SO, EPI_PKP>(a, nb, so);
-    else launch_one<TL, 12, SI, SO, EPI_RESTART>(a, nb, so);
+    if (epi == EPI_PLAIN) launch_one<TL, 12, SI, SO, EPI_PLAIN>(a, nb);
+    else if (epi == EPI_PKP) launch_one<TL, 12, SI, SO, EPI_PKP>(a, nb);
+    else launch_one<TL, 12, SI, SO, EPI_RESTART>(a, nb);
   } else {
-    if (epi == EPI_PLAIN) launch_one<TL, 18, SI, SO, EPI_PLAIN>(a, nb, so);
-    else if (epi == EPI_PKP) launch_one<TL, 18, SI, SO, EPI_PKP>(a, nb, so);
-    else launch_one<TL, 18, SI, SO, EPI_RESTART>(a, nb, so);
+    if (epi == EPI_PLAIN) launch_one<TL, 18, SI, SO, EPI_PLAIN>(a, nb);
+    else if (epi == EPI_PKP) launch_one<TL, 18, SI, SO, EPI_PKP>(a, nb);
+    else launch_one<TL, 18, SI, SO, EPI_RESTART>(a, nb);
   }
   HIP_CHECK(hipGetLastError());
   if (epi == EPI_PKP) reduce_finish(1, nb, a.rc, a.gate);
@@ -85,7 +81,7 @@ static DslashKArgs<TL, SI, SO> make_args(const LinksT<TL> &L, int cb, SO *out, c
 template <typename T>
 void launch_dslash(const LinksT<T> &L, int cb, T *out, const T *in, int epi, const DslashExtra &ex) {
   QPHIX_REQUIRE(L.fat[cb] && L.lng[cb] && in && out, "Dslash: links or vectors of this parity missing");
-  dispatch<T, T, T>(make_args<T, T, T>(L, cb, out, in, ex), L.recon, epi, ex.stream_out);
+  dispatch<T, T, T>(make_args<T, T, T>(L, cb, out, in, ex), L.recon, epi);
 }
 
 template void launch_dslash<double>(const LinksT<double> &, int, double *, const double *, int,

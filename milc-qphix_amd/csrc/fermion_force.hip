@@ -14,8 +14,8 @@
 //
 // Device work per call: Y_k by the FP64 (FP32) Dslash, then k_fermion_oprod,
 // one work-item per (site, mu, parity), accumulating groups of OPROD_GROUP
-// terms in FP64 registers; outputs stay in HBM in a coalesced SoA layout
-// [parity][mu][element][site] (double2) and are exported to the raw MILC
+// terms in FP64 registers; outputs stay in HBM in a coalesced layout
+// [parity][element][site][mu] (double2) and are exported to the raw MILC
 // layout only when the caller asks for host arrays.
 #include "dslash_kernel.h"
 #include "handles.h"
@@ -36,7 +36,7 @@ template <typename T> struct OprodArgs {
   double c[OPROD_GROUP];
   int nt;                   // terms in this group (1..OPROD_GROUP)
   int accumulate;           // add to the outputs instead of overwriting them
-  double2 *o1, *o3;         // [cb][mu][9][vh]
+  double2 *o1, *o3;         // [cb][9][vh][mu]
   Geo g;
 };
 
@@ -59,20 +59,30 @@ __device__ __forceinline__ void fwd_hops(const Geo &g, int i, int cb, int mu, in
   n3 = i + 3 * s - (c + 3 >= L ? P : 0);
 }
 
+// Work-item t of parity cb handles site t / 4 and direction t % 4: the four
+// directions of a site sit in neighbouring lanes, so psi(y) is loaded once per
+// site from L1 and the four outputs of each element are one contiguous 64 B
+// store ([cb][element][site][mu] layout).
+__device__ __forceinline__ size_t oprod_index(int cb, int e, int i, int mu, int vh) {
+  return ((size_t)(cb * 9 + e) * vh + i) * 4 + mu;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_fermion_oprod(const OprodArgs<T> a) {
   const int vh = a.g.vh;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int mu = blockIdx.y, cb = blockIdx.z;  // uniform per block
+  // XCD x (blocks are dispatched round-robin over the 8 XCDs) sweeps the x-th
+  // contiguous slab of sites, so neighbour spinors are reused from its L2
+  const int slot = block_slot(OPT_XCD);
+  const int t = slot * blockDim.x + threadIdx.x;
+  const int i = t >> 2, mu = t & 3, cb = blockIdx.y;
   if (i >= vh) return;
   int n1, n3;
   fwd_hops(a.g, i, cb, mu, n1, n3);
-  const size_t base = (size_t)((cb * 4 + mu) * 9) * vh + i;
   cplx<double> o1[9], o3[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
     if (a.accumulate) {
-      const double2 u = a.o1[base + (size_t)e * vh], w = a.o3[base + (size_t)e * vh];
+      const double2 u = a.o1[oprod_index(cb, e, i, mu, vh)], w = a.o3[oprod_index(cb, e, i, mu, vh)];
       o1[e] = cplx<double>{u.x, u.y};
       o3[e] = cplx<double>{w.x, w.y};
     } else {
@@ -107,8 +117,8 @@ __global__ void __launch_bounds__(256) k_fermion_oprod(const OprodArgs<T> a) {
   }
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
-    a.o1[base + (size_t)e * vh] = make_double2(o1[e].re, o1[e].im);
-    a.o3[base + (size_t)e * vh] = make_double2(o3[e].re, o3[e].im);
+    a.o1[oprod_index(cb, e, i, mu, vh)] = make_double2(o1[e].re, o1[e].im);
+    a.o3[oprod_index(cb, e, i, mu, vh)] = make_double2(o3[e].re, o3[e].im);
   }
 }
 
@@ -119,7 +129,7 @@ __global__ void __launch_bounds__(256) k_oprod_export(const double2 *o, int cb, 
   if (i >= vh) return;
   for (int mu = 0; mu < 4; ++mu)
     for (int e = 0; e < 9; ++e) {
-      const double2 v = o[(size_t)((cb * 4 + mu) * 9 + e) * vh + i];
+      const double2 v = o[oprod_index(cb, e, i, mu, vh)];
       raw[72 * (size_t)i + 18 * mu + 2 * e] = (T)v.x;
       raw[72 * (size_t)i + 18 * mu + 2 * e + 1] = (T)v.y;
     }
@@ -162,7 +172,8 @@ static bool oprod_device(const LinksT<T> &L, int nterms, const double *coef,
     a.o3 = o3;
     a.g = Geo{gg.nxh, gg.ny, gg.nz, gg.nt, gg.vh, gg.nxy, gg.nxyz};
     if (kernel_ms) HIP_CHECK(hipEventRecord(e0, ctx().stream));
-    hipLaunchKernelGGL(k_fermion_oprod<T>, dim3((vh + 255) / 256, 4, 2), dim3(256), 0, ctx().stream, a);
+    hipLaunchKernelGGL(k_fermion_oprod<T>, dim3((unsigned)((4 * (size_t)vh + 255) / 256), 2), dim3(256), 0,
+                       ctx().stream, a);
     HIP_CHECK(hipGetLastError());
     if (kernel_ms) {
       HIP_CHECK(hipEventRecord(e1, ctx().stream));

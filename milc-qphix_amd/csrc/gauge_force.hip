@@ -109,8 +109,17 @@ __device__ void walk_acc(const GFArgs<T> &a, const int *y0, const int *steps, in
   }
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs; renumbering them so that
+// XCD x sweeps the x-th contiguous slab of links keeps the paths' link loads
+// (up to 2 sites away) inside that XCD's L2 instead of spreading every
+// lattice region over all eight caches.
+__device__ __forceinline__ int gf_block_slot() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  return nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
 template <typename T> __global__ void __launch_bounds__(256) k_gauge_force(const GFArgs<T> a) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = gf_block_slot() * blockDim.x + threadIdx.x;
   const int nlink = 8 * a.g.vh;
   if (l >= nlink) return;
   const int cb = l / (4 * a.g.vh);

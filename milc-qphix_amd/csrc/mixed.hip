@@ -221,6 +221,7 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
     const int parity = todo[pass], other = 1 - parity;
     double *x = out->half[parity];
     const double *b = in->half[parity];
+    BatchGraph steps;  // this parity's batch of sloppy CG steps
     hs.iteration = 0;
     hs.active = 0;
     write_state(dst, hs);
@@ -264,15 +265,17 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
       }
       // sloppy CG steps until the device flags the next reliable update
       while (true) {
-        for (int k = 0; k < batch; ++k) {
-          sloppy_apply(Lf, parity, p, tmps, Ap, -msq_x4, dst, gate);
-          const ReduceCtl rc = reduce_ctl(dst, SOP_MX_RSQ);
-          hipLaunchKernelGGL(k_mx_step<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs,
-                             Ap, vh, rc, gate);
-          reduce_finish(1, nb, rc, gate);
-          hipLaunchKernelGGL(k_mx_axpy<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, vh,
-                             dst, gate);
-        }
+        steps.run([&]() {
+          for (int k = 0; k < batch; ++k) {
+            sloppy_apply(Lf, parity, p, tmps, Ap, -msq_x4, dst, gate);
+            const ReduceCtl rc = reduce_ctl(dst, SOP_MX_RSQ);
+            hipLaunchKernelGGL(k_mx_step<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs,
+                               Ap, vh, rc, gate);
+            reduce_finish(1, nb, rc, gate);
+            hipLaunchKernelGGL(k_mx_axpy<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, vh,
+                               dst, gate);
+          }
+        });
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(ctx().pinned, dst, HEAD, hipMemcpyDeviceToHost, ctx().stream));
         HIP_CHECK(hipStreamSynchronize(ctx().stream));

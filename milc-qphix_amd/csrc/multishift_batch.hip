@@ -34,6 +34,7 @@
 #include <chrono>
 #include <map>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -200,6 +201,14 @@ template <typename K> static int resident_grid(K kernel, int nlog) {
   return std::max(N_XCD, std::min(need, g));
 }
 
+// QPHIX_MRHS_K (1, 2 or 4), when set, replaces the tuned sources-per-work-item.
+static int mrhs_k_override(int tuned) {
+  const char *e = std::getenv("QPHIX_MRHS_K");
+  if (!e) return tuned;
+  const int k = std::atoi(e);
+  return k == 1 || k == 2 || k == 4 ? k : tuned;
+}
+
 // Logical 256-item blocks of a multi-RHS launch with K sources per work-item.
 inline int mrhs_nlog(int vh, int NR, int K) {
   return (int)(((size_t)vh * (NR / K) + DSLASH_BLOCK - 1) / DSLASH_BLOCK);
@@ -355,7 +364,7 @@ static double dslash_multi_api(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVec
   const MrhsTune &tune = autotune_mrhs(*L);
   if (tune.nr == 0) return -1.0;
   const int nr = n == 1 ? 1 : n == 2 ? 2 : n <= 4 ? 4 : 8;
-  const int K = tune.k[nr];
+  const int K = mrhs_k_override(tune.k[nr]);
   if (nr == 1) return dslash_multi<1>(L, out, in, n, cb, reps, K);
   if (nr == 2) return dslash_multi<2>(L, out, in, n, cb, reps, K);
   if (nr == 4) return dslash_multi<4>(L, out, in, n, cb, reps, K);
@@ -423,7 +432,7 @@ int QPHIX_D3_asqtad_invert_multi_rhs(QPHIX_info_t *info, QPHIX_D3_FermionLinksAs
   if (tune.nr == 0) return 0;                 // workspace allocation failed (error recorded)
   int nb = rhs_block > 0 ? rhs_block : tune.nr;
   while (nb > 1 && nb / 2 >= nrhs) nb /= 2;   // no block wider than the sources need
-  const int K = tune.k[nb];
+  const int K = mrhs_k_override(tune.k[nb]);
   switch (nb) {
   case 1: return invert_multi_rhs<1>(info, L, inv, res, mass, nshift, nrhs, out, in, K);
   case 2: return invert_multi_rhs<2>(info, L, inv, res, mass, nshift, nrhs, out, in, K);

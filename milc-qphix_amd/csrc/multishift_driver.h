@@ -163,22 +163,10 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
       msx_batch<NR, float>(Lf, parity, batch, p, xs, rs, Ap, tmp, j_low, offset_low, st, K, nbd, nbr,
                            nlog, nshift);
   };
-  hipGraphExec_t gexec = nullptr;
-  auto capture = [&]() {
-    if (gexec) HIP_CHECK(hipGraphExecDestroy(gexec));
-    gexec = nullptr;
-    if (!ctx().graphs) return;
-    // grid queries happen before capture
-    (void)mrhs_blocks<NR>(vh, K, Lf.recon, EPI_PLAIN);
-    (void)mrhs_blocks<NR, double>(vh, K64, L->recon, EPI_PLAIN);
-    hipGraph_t graph = nullptr;
-    HIP_CHECK(hipStreamBeginCapture(ctx().stream, hipStreamCaptureModeThreadLocal));
-    enqueue_batch();
-    HIP_CHECK(hipStreamEndCapture(ctx().stream, &graph));
-    HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-    HIP_CHECK(hipGraphDestroy(graph));
-  };
-  capture();
+  // grid queries (occupancy calculator) happen before any capture
+  (void)mrhs_blocks<NR>(vh, K, Lf.recon, EPI_PLAIN);
+  (void)mrhs_blocks<NR, double>(vh, K64, L->recon, EPI_PLAIN);
+  BatchGraph steps;  // re-captured when the chunk drops to FP32
 
   // FP64 true residual b - (M + sigma_low) x_low of source r into r64; returns
   // its SolverState (rsq, final_rsq = rsq / |b|^2).
@@ -223,8 +211,7 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     bool any = false;
     for (int r = 0; r < nrhs; ++r) any |= !done[r];
     if (!any) break;
-    if (gexec) HIP_CHECK(hipGraphLaunch(gexec, ctx().stream));
-    else enqueue_batch();
+    steps.run(enqueue_batch);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(ctx().pinned, st, NR * sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
     HIP_CHECK(hipStreamSynchronize(ctx().stream));
@@ -250,7 +237,7 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
       dev_free(w64);
       w64 = nullptr;
       hi = false;
-      capture();
+      steps.reset();
       continue;
     }
     for (int r = 0; r < nrhs; ++r) {
@@ -262,7 +249,6 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     }
   }
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
-  if (gexec) HIP_CHECK(hipGraphExecDestroy(gexec));
   dev_free(w64);
   return true;
 }

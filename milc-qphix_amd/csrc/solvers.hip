@@ -59,7 +59,6 @@ template <typename T> static void dslash_api(LinksT<T> *L, VectorT<T> *out, Vect
   QPHIX_REQUIRE(out->half[cb] && in->half[1 - cb],
                 "asqtad_dslash: out must hold the output parity and in the opposite one");
   DslashExtra ex;
-  ex.stream_out = true;  // the result goes back to the caller
   launch_dslash<T>(*L, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
   sync_if_needed();
 }
@@ -75,7 +74,6 @@ static double dslash_time(LinksT<T> *L, VectorT<T> *out, VectorT<T> *in, int par
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
   DslashExtra ex;
-  ex.stream_out = true;  // times the stand-alone asqtad_dslash
   HIP_CHECK(hipEventRecord(e0, ctx().stream));
   for (int r = 0; r < reps; ++r) launch_dslash<T>(*L, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
   HIP_CHECK(hipEventRecord(e1, ctx().stream));
@@ -133,6 +131,7 @@ static int invert(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *inv, QPH
     const int parity = todo[pass], other = 1 - parity;
     T *xd = out->half[parity];
     const T *bs = in->half[parity];
+    BatchGraph steps;  // this parity's batch of CG steps
     total_iters += iteration;
     iteration = 0;
     nrestart = 0;
@@ -171,20 +170,22 @@ static int invert(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *inv, QPH
       set_int(gate, 1);
       // CG steps until the device clears the gate (restart rule met)
       while (true) {
-        for (int s = 0; s < batch; ++s) {
-          DslashExtra a1;
-          a1.gate = gate;
-          launch_dslash<T>(*L, other, tmp, p, EPI_PLAIN, a1);
-          DslashExtra a2;
-          a2.gate = gate;
-          a2.s = -hs.msq_x4;
-          a2.aux = p;
-          a2.state = dst;
-          a2.scalar_op = SOP_CG_PKP;
-          launch_dslash<T>(*L, parity, ttt, tmp, EPI_PKP, a2);
-          blas_cg_update<T>(xd, p, r, ttt, dst, gate);
-          blas_cg_axpy<T>(r, p, dst, gate);
-        }
+        steps.run([&]() {
+          for (int s = 0; s < batch; ++s) {
+            DslashExtra a1;
+            a1.gate = gate;
+            launch_dslash<T>(*L, other, tmp, p, EPI_PLAIN, a1);
+            DslashExtra a2;
+            a2.gate = gate;
+            a2.s = -hs.msq_x4;
+            a2.aux = p;
+            a2.state = dst;
+            a2.scalar_op = SOP_CG_PKP;
+            launch_dslash<T>(*L, parity, ttt, tmp, EPI_PKP, a2);
+            blas_cg_update<T>(xd, p, r, ttt, dst, gate);
+            blas_cg_axpy<T>(r, p, dst, gate);
+          }
+        });
         state_read(dst, hs);
         if (hs.active == 0) break;
       }
@@ -297,8 +298,7 @@ static int invert_multi(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *in
     state_upload(dst, hs);
     T *plow = hp[nshift + j_low];
     int launched = 0;
-    while (launched < max_cg) {
-      const int nstep = std::min(batch, max_cg - launched);
+    auto enqueue = [&](int nstep) {
       for (int s = 0; s < nstep; ++s) {
         DslashExtra a1;
         a1.gate = gate;
@@ -313,6 +313,12 @@ static int invert_multi(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *in
         blas_ms_update_r<T>(r, ttt, dst, gate);
         blas_ms_update_pm<T>(dptr, dptr + nshift, r, nshift, dst, gate);
       }
+    };
+    BatchGraph steps;  // full batches; the last one, cut at max_cg, is enqueued directly
+    while (launched < max_cg) {
+      const int nstep = std::min(batch, max_cg - launched);
+      if (nstep == batch) steps.run([&]() { enqueue(batch); });
+      else enqueue(nstep);
       launched += nstep;
       state_read(dst, hs);
       if (hs.active == 0) break;

@@ -12,8 +12,18 @@ from pyqphix import fields as F
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=[None, "1", "2", "4"], ids=["tunedK", "K1", "K2", "K4"])
+def sources_per_item(request, monkeypatch):
+    """Runs a test with the autotuned sources-per-work-item, then each K forced."""
+    if request.param is None:
+        monkeypatch.delenv("QPHIX_MRHS_K", raising=False)
+    else:
+        monkeypatch.setenv("QPHIX_MRHS_K", request.param)
+    return request.param
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 8])
-def test_dslash_multi_matches_oracle(n):
+def test_dslash_multi_matches_oracle(n, sources_per_item):
     dims = (8, 8, 8, 8)
     s = lattice_session(dims)
     lat = F.Lattice(dims)
@@ -40,7 +50,7 @@ def true_rsq(orc, lat, x, b, mass, cb):
 
 
 @pytest.mark.parametrize("nrhs,block", [(3, 0), (4, 4), (8, 8), (2, 1)])
-def test_multi_rhs_multishift(nrhs, block):
+def test_multi_rhs_multishift(nrhs, block, sources_per_item):
     dims = (8, 8, 8, 8)
     s = lattice_session(dims)
     lat = F.Lattice(dims)
@@ -52,7 +62,7 @@ def test_multi_rhs_multishift(nrhs, block):
     bs = [F.to_raw(F.gaussian_source(lat, 200 + r, parity=0)) for r in range(nrhs)]
     vb = [A.Vector(s, b, A.EVEN) for b in bs]
     vx = [[A.Vector(s, None, A.EVEN) for _ in masses] for _ in bs]
-    print(f"[case] multi_rhs nrhs={nrhs} block={block}", flush=True)
+    print(f"[case] multi_rhs nrhs={nrhs} block={block} K={sources_per_item}", flush=True)
     ret, info, res = A.invert_multi_rhs(L, vx, vb, masses, A.EVEN, resid, niter=1000,
                                         nrestart=2, rhs_block=block)
     print(f"[case] ret={ret} iters={[[res[r][j].final_iter for j in range(len(masses))] for r in range(nrhs)]}",
