@@ -35,6 +35,19 @@ def site_bytes(real_bytes, n_long, spinor_bytes):
     return real_bytes * (8 * 18 + 8 * n_long) + sign + 2 * spinor_bytes
 
 
+def measured_traffic():
+    """HBM bytes per launch of the FP64 plain Dslash (FETCH_SIZE + WRITE_SIZE, each
+    scaled by the copy-kernel factor) from the committed rocprofv3 summary of this
+    benchmark's Dslash run, or None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "dslash_32x4_counters.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return int(round(d["fetch_bytes"] + d["write_bytes"]))
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -241,6 +254,7 @@ def combined_48(A, F, dims=(48, 48, 48, 96), nrhs=4, nshift=16, mass=0.0102):
     lat = F.Lattice(dims)
     log(f"[bench] combined {dims}: generating links")
     fat, lng = _weak_links_gpu(lat, F, 51, 0.3)
+    log("[bench]   raw link arrays")
     raw = F.raw_link_set(lat, fat, lng)
     del fat, lng
     sig = np.logspace(np.log10(4 * mass * mass), 0.0, nshift)
@@ -249,6 +263,7 @@ def combined_48(A, F, dims=(48, 48, 48, 96), nrhs=4, nshift=16, mass=0.0102):
     out = {"lattice": list(dims), "nrhs": nrhs, "nshift": nshift, "mass": mass,
            "sigma": [float(x) for x in sig]}
     with A.Session(dims) as s:
+        log("[bench]   links to the device")
         L = A.Links(s, *raw)
         del raw
         bs = [A.Vector(s, F.to_raw(F.gaussian_source(lat, 5000 + r, parity=0)), A.EVEN)
@@ -273,8 +288,9 @@ def combined_48(A, F, dims=(48, 48, 48, 96), nrhs=4, nshift=16, mass=0.0102):
         log("[bench] combined: FP64 multi-shift per source")
         x64 = [[A.Vector(s, None, A.EVEN) for _ in masses] for _ in bs]
         t = time.perf_counter()
-        for b, row in zip(bs, x64):
+        for k, (b, row) in enumerate(zip(bs, x64)):
             A.invert_multi(L, row, b, masses, A.EVEN, 1e-20, niter=20000, nrestart=2)
+            log(f"[bench]   FP64 source {k} done ({time.perf_counter() - t:.1f}s)")
         t64 = time.perf_counter() - t
         flat = [x for row in x64 for x in row]
         t_force64, _ = A.fermion_force_time(L, cf, flat, reps=1)
@@ -399,7 +415,7 @@ def main():
                                "weak-field links eps=0.3, FP64, Naik links as 12 reals",
                    "lattice": list(DIMS), "sites_per_step": lat.vh},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic()},
         "cpu_baseline": cpu,
         "extra": extra,
     }
