@@ -13,6 +13,12 @@ template <typename T> void blas_norm2(const T *v, SolverState *st, int op);
 template <typename T>
 void blas_cg_update(T *x, const T *p, T *r, const T *ttt, SolverState *st, const int *gate);
 template <typename T> void blas_cg_axpy(const T *r, T *p, const SolverState *st, const int *gate);
+// One CG step after a p.Ap Dslash launched with consumer_finalize (nb_pkp
+// blocks of partials): update x, r and |r|^2, then p, each kernel summing the
+// previous one's partials itself (no finalize launches).
+template <typename T>
+void blas_cg_step_c(T *x, T *p, T *r, const T *ttt, SolverState *st, int *gate, int nb_pkp);
+double *partials_b();
 template <typename T> void blas_ms_update_r(T *r, const T *ttt, SolverState *st, const int *gate);
 template <typename T>
 void blas_ms_update_pm(T *const *x, T *const *p, const T *r, int nshift, const SolverState *st,

@@ -71,6 +71,100 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   Spin<T>::store(p, i, pv);
 }
 
+// CG step with consumer-side reductions (the arithmetic of SOP_CG_PKP and
+// k_cg_update): every block derives a = -rsq / pkp from the p.Ap partials of
+// the preceding Dslash; x += a p, r += a Ap; block 0 records pkp, a, oldrsq
+// and the iteration (fields no other block of this kernel reads); |r|^2 and
+// the relative-residue sum go to out_partials.
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_cg_update_c(T *x, const T *p, T *r, const T *ttt, int vh, SolverState *st,
+                  const double *pkp_partials, int nb_pkp, double *out_partials, const int *gate) {
+  if (!gate_open(gate)) return;
+  double pk[1];
+  sum_partials<1>(pkp_partials, nb_pkp, pk);
+  const double rsq = st->rsq;
+  const double a = rnd(st, -rsq / pk[0]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->oldrsq = rsq;
+    st->pkp = pk[0];
+    st->iteration++;
+    st->a = a;
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[2] = {0.0, 0.0};
+  if (i < vh) {
+    using C = typename Spin<T>::C;
+    const C ac = (C)a;
+    cplx<C> xv[3], pv[3], rv[3], tv[3];
+    Spin<T>::load(x, i, xv);
+    Spin<T>::load(p, i, pv);
+    Spin<T>::load(r, i, rv);
+    Spin<T>::load(ttt, i, tv);
+    double num = 0, den = 0;
+    for (int c = 0; c < 3; ++c) {
+      xv[c].re = xv[c].re + ac * pv[c].re;
+      xv[c].im = xv[c].im + ac * pv[c].im;
+      rv[c].re = rv[c].re + ac * tv[c].re;
+      rv[c].im = rv[c].im + ac * tv[c].im;
+      num += (double)rv[c].re * rv[c].re + (double)rv[c].im * rv[c].im;
+      den += (double)xv[c].re * xv[c].re + (double)xv[c].im * xv[c].im;
+    }
+    Spin<T>::store(x, i, xv);
+    Spin<T>::store(r, i, rv);
+    s[0] = num;
+    s[1] = num == 0 ? 0.0 : (den == 0 ? 1.0 : num / den);
+  }
+  block_sum<2>(s);
+  if (threadIdx.x == 0) {
+    out_partials[blockIdx.x] = s[0];
+    out_partials[(size_t)gridDim.x + blockIdx.x] = s[1];
+  }
+}
+
+// p = r + b p, with b and the stopping rule of SOP_CG_RSQ2 evaluated in every
+// block from the update's partials; block 0 records them and closes the gate
+// when the reference's inner loop would end (every block then skips the
+// update, whichever value of the gate it read).
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_cg_axpy_c(const T *r, T *p, int vh, SolverState *st, const double *partials, int nb, int *gate) {
+  if (!gate_open(gate)) return;
+  double v[2];
+  sum_partials<2>(partials, nb, v);
+  const double rsq = v[0];
+  // relrsq keeps its previous value when the relative residue is off
+  const double relrsq = st->relrsqmin > 0 ? rnd(st, sqrt(2.0 * v[1] / st->vol)) : st->relrsq;
+  const double size_r = rnd(st, rsq) / st->src_norm;
+  const double size_relr = rnd(st, relrsq);
+  const double b = rnd(st, rnd(st, rsq) / st->oldrsq);
+  const bool conv = (st->rsqmin <= 0 || st->rsqmin > size_r) &&
+                    (st->relrsqmin <= 0 || st->relrsqmin > size_relr);
+  const bool stop = st->iteration % st->niter == 0 || conv;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rsq = rsq;
+    st->relrsq = relrsq;
+    st->size_r = size_r;
+    st->size_relr = size_relr;
+    st->final_rsq = size_r;
+    st->b = b;
+    if (stop) *gate = 0;
+  }
+  if (stop) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  using C = typename Spin<T>::C;
+  const C bc = (C)b;
+  cplx<C> rv[3], pv[3];
+  Spin<T>::load(r, i, rv);
+  Spin<T>::load(p, i, pv);
+  for (int c = 0; c < 3; ++c) {
+    pv[c].re = rv[c].re + bc * pv[c].re;
+    pv[c].im = rv[c].im + bc * pv[c].im;
+  }
+  Spin<T>::store(p, i, pv);
+}
+
 // multi-shift: r += beta_low ttt ; reduce |r|^2
 template <typename T>
 __global__ void __launch_bounds__(BLAS_BLOCK)
@@ -194,6 +288,20 @@ template <typename T> void blas_cg_axpy(const T *r, T *p, const SolverState *st,
   HIP_CHECK(hipGetLastError());
 }
 
+// Second partials region: consumer-side reductions read one region while
+// the consuming kernel writes its own partials into the other.
+double *partials_b() { return ctx().partials + 2 * (size_t)ctx().max_blocks; }
+
+template <typename T>
+void blas_cg_step_c(T *x, T *p, T *r, const T *ttt, SolverState *st, int *gate, int nb_pkp) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_cg_update_c<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, x, p, r, ttt, vh, st,
+                     ctx().partials, nb_pkp, partials_b(), gate);
+  hipLaunchKernelGGL(k_cg_axpy_c<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, r, p, vh, st,
+                     partials_b(), nb(vh), gate);
+  HIP_CHECK(hipGetLastError());
+}
+
 template <typename T> void blas_ms_update_r(T *r, const T *ttt, SolverState *st, const int *gate) {
   const int vh = ctx().g.vh;
   const ReduceCtl rc = reduce_ctl(st, SOP_MS_RSQ2);
@@ -228,6 +336,7 @@ void set_int(int *p, int v) {
 #define INST(T)                                                                                \
   template void blas_norm2<T>(const T *, SolverState *, int);                                  \
   template void blas_cg_update<T>(T *, const T *, T *, const T *, SolverState *, const int *); \
+  template void blas_cg_step_c<T>(T *, T *, T *, const T *, SolverState *, int *, int);         \
   template void blas_cg_axpy<T>(const T *, T *, const SolverState *, const int *);             \
   template void blas_ms_update_r<T>(T *, const T *, SolverState *, const int *);               \
   template void blas_ms_update_pm<T>(T *const *, T *const *, const T *, int,                   \

@@ -174,6 +174,7 @@ struct DslashExtra {
   void *state = nullptr;       // device SolverState updated by the last block
   int scalar_op = 0;           // ScalarOp run by the last block
   const int *gate = nullptr;   // kernel is a no-op when *gate == 0
+  bool consumer_finalize = false;  // reducing epilogues: leave the partials to the next kernel
 };
 
 template <typename T>

@@ -36,7 +36,7 @@ static void launch_one(const DslashKArgs<TL, SI, SO> &a, int nblocks) {
 }
 
 template <typename TL, typename SI, typename SO>
-static void dispatch(const DslashKArgs<TL, SI, SO> &a, int recon, int epi) {
+static void dispatch(const DslashKArgs<TL, SI, SO> &a, int recon, int epi, bool consumer = false) {
   const int nb = (a.g.vh + DSLASH_BLOCK - 1) / DSLASH_BLOCK;
   if (recon == 12) {
     if (epi == EPI_PLAIN) launch_one<TL, 12, SI, SO, EPI_PLAIN>(a, nb);
@@ -48,6 +48,7 @@ static void dispatch(const DslashKArgs<TL, SI, SO> &a, int recon, int epi) {
     else launch_one<TL, 18, SI, SO, EPI_RESTART>(a, nb);
   }
   HIP_CHECK(hipGetLastError());
+  if (consumer) return;  // the next kernel sums the partials itself
   if (epi == EPI_PKP) reduce_finish(1, nb, a.rc, a.gate);
   else if (epi == EPI_RESTART) reduce_finish(2, nb, a.rc, a.gate);
 }
@@ -81,7 +82,7 @@ static DslashKArgs<TL, SI, SO> make_args(const LinksT<TL> &L, int cb, SO *out, c
 template <typename T>
 void launch_dslash(const LinksT<T> &L, int cb, T *out, const T *in, int epi, const DslashExtra &ex) {
   QPHIX_REQUIRE(L.fat[cb] && L.lng[cb] && in && out, "Dslash: links or vectors of this parity missing");
-  dispatch<T, T, T>(make_args<T, T, T>(L, cb, out, in, ex), L.recon, epi);
+  dispatch<T, T, T>(make_args<T, T, T>(L, cb, out, in, ex), L.recon, epi, ex.consumer_finalize);
 }
 
 template void launch_dslash<double>(const LinksT<double> &, int, double *, const double *, int,
@@ -94,8 +95,12 @@ template void launch_dslash<float>(const LinksT<float> &, int, float *, const fl
 void launch_dslash_half(const LinksT<float> &L, int cb, HalfSite *out_h, float *out_f,
                         const HalfSite *in, int epi, const DslashExtra &ex) {
   QPHIX_REQUIRE(L.fat[cb] && L.lng[cb] && in && (out_h || out_f), "Dslash: links or vectors of this parity missing");
-  if (out_h) dispatch<float, HalfSite, HalfSite>(make_args<float, HalfSite, HalfSite>(L, cb, out_h, in, ex), L.recon, epi);
-  else dispatch<float, HalfSite, float>(make_args<float, HalfSite, float>(L, cb, out_f, in, ex), L.recon, epi);
+  if (out_h)
+    dispatch<float, HalfSite, HalfSite>(make_args<float, HalfSite, HalfSite>(L, cb, out_h, in, ex), L.recon, epi,
+                                        ex.consumer_finalize);
+  else
+    dispatch<float, HalfSite, float>(make_args<float, HalfSite, float>(L, cb, out_f, in, ex), L.recon, epi,
+                                     ex.consumer_finalize);
 }
 
 } // namespace qphix

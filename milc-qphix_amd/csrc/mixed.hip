@@ -9,6 +9,7 @@
 // reductions.  Each reliable update and each sloppy step counts as one
 // iteration (the reference's counting, C9).
 #include "blas.h"
+#include "dslash_kernel.h"
 #include "handles.h"
 #include "qphix_mi355x.h"
 
@@ -55,6 +56,83 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   const float be = (float)st->mx_beta;
+  cplx<float> rv[3], pv[3];
+  Spin<float>::load(rs, i, rv);
+  Spin<P>::load(p, i, pv);
+  for (int c = 0; c < 3; ++c) {
+    pv[c].re = rv[c].re + be * pv[c].re;
+    pv[c].im = rv[c].im + be * pv[c].im;
+  }
+  Spin<P>::store(p, i, pv);
+}
+
+// The sloppy step with consumer-side reductions (the arithmetic of SOP_MX_PAP
+// and k_mx_step): every block derives alpha = -|r|^2 / p.(-Mp) from the
+// preceding Dslash's partials; block 0 records pkp, alpha, the iteration and
+// the old |r|^2 (fields no other block of this kernel reads).
+template <typename P>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_mx_step_c(float *xs, const P *p, float *rs, const float *Ap, int vh, SolverState *st,
+                const double *pkp_partials, int nb_pkp, double *out_partials, const int *gate) {
+  if (!gate_open(gate)) return;
+  double pk[1];
+  sum_partials<1>(pkp_partials, nb_pkp, pk);
+  const double rsq = st->mx_rsq;
+  const double alpha = -rsq / pk[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pkp = pk[0];
+    st->iteration++;
+    st->mx_alpha = alpha;
+    st->mx_rsq_old = rsq;
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[1] = {0.0};
+  if (i < vh) {
+    const float al = (float)alpha;
+    cplx<float> xv[3], pv[3], rv[3], av[3];
+    Spin<float>::load(xs, i, xv);
+    Spin<P>::load(p, i, pv);
+    Spin<float>::load(rs, i, rv);
+    Spin<float>::load(Ap, i, av);
+    for (int c = 0; c < 3; ++c) {
+      xv[c].re += al * pv[c].re;
+      xv[c].im += al * pv[c].im;
+      rv[c].re += al * av[c].re;
+      rv[c].im += al * av[c].im;
+      s[0] += (double)rv[c].re * rv[c].re + (double)rv[c].im * rv[c].im;
+    }
+    Spin<float>::store(xs, i, xv);
+    Spin<float>::store(rs, i, rv);
+  }
+  block_sum<1>(s);
+  if (threadIdx.x == 0) out_partials[blockIdx.x] = s[0];
+}
+
+// p = r_s + beta p, with beta and the reliable-update test of SOP_MX_RSQ
+// evaluated in every block from the step's partials; block 0 records them and
+// closes the gate when an update is due (every block then skips, whichever
+// value of the gate it read; the running maximum is updated idempotently).
+template <typename P>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_mx_axpy_c(const float *rs, P *p, int vh, SolverState *st, const double *partials, int nb, int *gate) {
+  if (!gate_open(gate)) return;
+  double v[1];
+  sum_partials<1>(partials, nb, v);
+  const double rnew = v[0];
+  const double beta = rnew / st->mx_rsq_old;
+  const double r = sqrt(rnew);
+  const double maxr = fmax(st->mx_maxr, r);
+  const bool update = r < st->mx_delta * maxr || rnew < st->rsqstop || st->iteration >= st->max_cg;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->mx_beta = beta;
+    st->mx_rsq = rnew;
+    st->mx_maxr = maxr;
+    if (update) *gate = 0;
+  }
+  if (update) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  const float be = (float)beta;
   cplx<float> rv[3], pv[3];
   Spin<float>::load(rs, i, rv);
   Spin<P>::load(p, i, pv);
@@ -153,6 +231,7 @@ static void write_state(SolverState *d, const SolverState &h) {
 }
 
 // Sloppy operator application Ap = -(4m^2 - D D) p, reducing p.Ap.
+// The p.Ap partials are left to the following k_mx_step_c (consumer_finalize).
 static void sloppy_apply(const LinksT<float> &Lf, int parity, float *p, float *tmp, float *Ap,
                          double s, SolverState *st, const int *gate) {
   DslashExtra e1;
@@ -164,6 +243,7 @@ static void sloppy_apply(const LinksT<float> &Lf, int parity, float *p, float *t
   e2.aux = p;
   e2.state = st;
   e2.scalar_op = SOP_MX_PAP;
+  e2.consumer_finalize = true;
   launch_dslash<float>(Lf, parity, Ap, tmp, EPI_PKP, e2);
 }
 
@@ -178,6 +258,7 @@ static void sloppy_apply(const LinksT<float> &Lf, int parity, HalfSite *p, HalfS
   e2.aux = p;
   e2.state = st;
   e2.scalar_op = SOP_MX_PAP;
+  e2.consumer_finalize = true;
   launch_dslash_half(Lf, parity, nullptr, Ap, tmp, EPI_PKP, e2);
 }
 
@@ -204,6 +285,7 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
   int *gate = &dst->active;
   const double msq_x4 = 4.0 * mass * mass;
   const int nb = (vh + BLAS_BLOCK - 1) / BLAS_BLOCK;
+  const int nb_dslash = (vh + DSLASH_BLOCK - 1) / DSLASH_BLOCK;
   const int nbn = (int)((nv + BLAS_BLOCK - 1) / BLAS_BLOCK);
   const char *de = std::getenv("QPHIX_MIXED_DELTA");
   const int batch = 16;
@@ -268,12 +350,10 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
         steps.run([&]() {
           for (int k = 0; k < batch; ++k) {
             sloppy_apply(Lf, parity, p, tmps, Ap, -msq_x4, dst, gate);
-            const ReduceCtl rc = reduce_ctl(dst, SOP_MX_RSQ);
-            hipLaunchKernelGGL(k_mx_step<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs,
-                               Ap, vh, rc, gate);
-            reduce_finish(1, nb, rc, gate);
-            hipLaunchKernelGGL(k_mx_axpy<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, vh,
-                               dst, gate);
+            hipLaunchKernelGGL(k_mx_step_c<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs, Ap,
+                               vh, dst, ctx().partials, nb_dslash, partials_b(), gate);
+            hipLaunchKernelGGL(k_mx_axpy_c<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, vh, dst,
+                               partials_b(), nb, gate);
           }
         });
         HIP_CHECK(hipGetLastError());

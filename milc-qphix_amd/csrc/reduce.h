@@ -260,6 +260,38 @@ template <int NV> __device__ __forceinline__ void grid_reduce(double *v, const R
 
 constexpr int FINALIZE_BLOCK = 1024;
 
+// Consumer-side finalize.  A kernel that needs the previous kernel's
+// reduction sums its per-block partials ([NV][nb]) itself, in one fixed order,
+// so every block obtains the same value bit for bit and no separate finalize
+// launch (and its dependency gap) is needed.  Result valid in every thread.
+template <int NV> __device__ __forceinline__ void sum_partials(const double *partials, int nb, double *v) {
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] += partials[(size_t)k * nb + b];
+  }
+  block_sum<NV>(s);
+  __shared__ double bc[NV];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) bc[k] = s[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = bc[k];
+}
+
+// Block-uniform read of a gate that block 0 of the same kernel may clear:
+// thread 0 reads it once and the block follows that value.
+__device__ __forceinline__ bool gate_open(const int *gate) {
+  __shared__ int g;
+  if (threadIdx.x == 0) g = *gate;
+  __syncthreads();
+  return g != 0;
+}
+
 template <int NV>
 __global__ void __launch_bounds__(FINALIZE_BLOCK)
     k_finalize(const double *partials, int nb, ReduceCtl rc, const int *gate) {

@@ -8,6 +8,7 @@
 // into no-ops once the reference would leave its inner loop, so iteration
 // counts are exact.
 #include "blas.h"
+#include "dslash_kernel.h"
 #include "handles.h"
 #include "qphix_mi355x.h"
 
@@ -124,6 +125,7 @@ static int invert(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *inv, QPH
   else { todo[0] = inv->parity == QPHIX_EVEN ? 0 : 1; ntodo = 1; }
 
   const int batch = steps_per_batch();
+  const int nb_dslash = (vh + DSLASH_BLOCK - 1) / DSLASH_BLOCK;
   int total_iters = 0, iteration = 0, nrestart = 0;
   info->final_flop = 0;
   info->status = QPHIX_SUCCESS;
@@ -181,9 +183,9 @@ static int invert(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *inv, QPH
             a2.aux = p;
             a2.state = dst;
             a2.scalar_op = SOP_CG_PKP;
+            a2.consumer_finalize = true;
             launch_dslash<T>(*L, parity, ttt, tmp, EPI_PKP, a2);
-            blas_cg_update<T>(xd, p, r, ttt, dst, gate);
-            blas_cg_axpy<T>(r, p, dst, gate);
+            blas_cg_step_c<T>(xd, p, r, ttt, dst, gate, nb_dslash);
           }
         });
         state_read(dst, hs);
