@@ -137,13 +137,20 @@ template <typename T> struct VectorT {
   int parity = 0;
 };
 
-// Internal gauge field (8 slots per even site, reference qphix_d3.cpp:177-343)
-// and momentum (8 reals per link, C13 mapping) kept in raw per-parity order on
-// the device: gauge[cb][i][mu][3][3][2], mom[cb][i][mu][8].
+// Internal gauge field (the reference keeps 8 slots per even site,
+// qphix_d3.cpp:177-343) and momentum (8 reals per link, C13 mapping): the
+// gauge field is complex-major per parity (gauge_elem), the momentum in raw
+// per-parity order mom[cb][i][mu][8].
+// Gauge links per parity, complex-major: element e (row-major 3x3) of the link
+// in direction mu at site i is the complex number at index (mu*9 + e)*vh + i,
+// so lanes at consecutive sites read consecutive complexes.
 template <typename T> struct GaugeT {
   T *u[2] = {nullptr, nullptr};
   int parity = 0;
 };
+__host__ __device__ __forceinline__ size_t gauge_elem(int mu, int e, int i, int vh) {
+  return (size_t)(mu * 9 + e) * vh + i;
+}
 template <typename T> struct ForceT {
   T *h[2] = {nullptr, nullptr};
   int parity = 0;

@@ -402,6 +402,29 @@ template <typename T> static void backcheck(const LinksT<T> *L, double out[4]) {
   dev_free(d);
 }
 
+// Raw gauge links of one parity ([i][mu][3][3][re,im]) <-> complex-major layout.
+template <typename T> __global__ void k_gauge_in(T *dst, const T *raw, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  for (int mu = 0; mu < 4; ++mu)
+    for (int e = 0; e < 9; ++e) {
+      const size_t d = 2 * gauge_elem(mu, e, i, vh);
+      dst[d] = raw[72 * (size_t)i + 18 * mu + 2 * e];
+      dst[d + 1] = raw[72 * (size_t)i + 18 * mu + 2 * e + 1];
+    }
+}
+
+template <typename T> __global__ void k_gauge_out(T *raw, const T *src, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  for (int mu = 0; mu < 4; ++mu)
+    for (int e = 0; e < 9; ++e) {
+      const size_t d = 2 * gauge_elem(mu, e, i, vh);
+      raw[72 * (size_t)i + 18 * mu + 2 * e] = src[d];
+      raw[72 * (size_t)i + 18 * mu + 2 * e + 1] = src[d + 1];
+    }
+}
+
 // ------------------------------------------------------------------ G, F
 // Gauge field: all links are always loaded (the reference reads both raw
 // blocks whatever the parity, qphix_d3.cpp:198-200); EVENODD is recorded as
@@ -411,24 +434,38 @@ template <typename T> static typename Handles<T>::G *create_G(const T *raw, int 
   QPHIX_REQUIRE(raw, "create_G: null array", nullptr);
   QPHIX_REQUIRE(evenodd == QPHIX_EVENODD || evenodd == QPHIX_EVEN || evenodd == QPHIX_ODD,
                 "create_G: bad parity", nullptr);
-  const size_t n = 72 * (size_t)ctx().g.vh;
+  const int vh = ctx().g.vh;
+  const size_t n = 72 * (size_t)vh;
+  T *stage = (T *)dev_alloc(sizeof(T) * n);
+  if (!stage) return nullptr;
   auto *G = new typename Handles<T>::G;
   G->parity = evenodd == QPHIX_EVENODD ? QPHIX_EVEN : evenodd;
   for (int cb = 0; cb < 2; ++cb) {
     G->u[cb] = (T *)dev_alloc(sizeof(T) * n);
-    h2d(G->u[cb], raw + (cb ? n : 0), sizeof(T) * n);
+    h2d(stage, raw + (cb ? n : 0), sizeof(T) * n);
+    hipLaunchKernelGGL(k_gauge_in<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, G->u[cb], stage, vh);
+    HIP_CHECK(hipGetLastError());
+    sync();
   }
-  sync();
+  dev_free(stage);
   return G;
 }
 
 template <typename T> static void extract_G(T *raw, const typename Handles<T>::G *G, int evenodd) {
   QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called:extract_G");
   QPHIX_REQUIRE(raw && G, "extract_G: null argument");
-  const size_t n = 72 * (size_t)ctx().g.vh;
-  for (int cb = 0; cb < 2; ++cb)
-    if (evenodd & (cb ? QPHIX_ODD : QPHIX_EVEN)) d2h(raw + (cb ? n : 0), G->u[cb], sizeof(T) * n);
-  sync();
+  const int vh = ctx().g.vh;
+  const size_t n = 72 * (size_t)vh;
+  T *stage = (T *)dev_alloc(sizeof(T) * n);
+  if (!stage) return;
+  for (int cb = 0; cb < 2; ++cb) {
+    if (!(evenodd & (cb ? QPHIX_ODD : QPHIX_EVEN))) continue;
+    hipLaunchKernelGGL(k_gauge_out<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, stage, G->u[cb], vh);
+    HIP_CHECK(hipGetLastError());
+    d2h(raw + (cb ? n : 0), stage, sizeof(T) * n);
+    sync();
+  }
+  dev_free(stage);
 }
 
 template <typename T> static void destroy_G(typename Handles<T>::G *G) {

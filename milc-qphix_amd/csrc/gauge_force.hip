@@ -15,11 +15,14 @@
 // MI355X form: gather, not scatter -- one work-item per link computes the 48
 // staples through its link (6 plaquette, 18 rectangle, 24 parallelogram) by
 // walking link paths from L2-resident neighbours, in FP64, and writes only its
-// own momentum.  No atomics, no second pass.
+// own momentum.  No atomics, no second pass.  The links are stored
+// complex-major (gauge_elem) and a wavefront's lanes are consecutive sites of
+// one direction, so every step of a walk is a coalesced load.
 #include "blas.h"
 #include "handles.h"
 
 #include <chrono>
+#include <type_traits>
 
 namespace qphix {
 
@@ -69,25 +72,57 @@ __device__ __forceinline__ void load_link(const GFArgs<T> &a, const int *y, int 
   }
   const int p = (c[0] + c[1] + c[2] + c[3]) & 1;
   const int i = (c[0] >> 1) + a.g.nxh * (c[1] + a.g.L[1] * (c[2] + a.g.L[2] * c[3]));
-  const T *s = a.u[p] + 72 * (size_t)i + 18 * d;
+  // complex-major storage: one 2 x T load per element, coalesced across lanes
+  // that walk the same loop from consecutive sites
+  using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  const T2 *s = reinterpret_cast<const T2 *>(a.u[p]);
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) {
       const int src = dag ? 3 * cc + r : 3 * r + cc;
-      m.re[3 * r + cc] = (double)s[2 * src];
-      m.im[3 * r + cc] = dag ? -(double)s[2 * src + 1] : (double)s[2 * src + 1];
+      const T2 v = s[gauge_elem(d, src, i, a.g.vh)];
+      m.re[3 * r + cc] = (double)v.x;
+      m.im[3 * r + cc] = dag ? -(double)v.y : (double)v.y;
     }
+}
+
+// r <- r * m, one row at a time (one row of temporaries instead of a third
+// matrix: the walk keeps two 3x3 FP64 matrices live, which sets occupancy).
+__device__ __forceinline__ void dm_rmul(DM3 &r, const DM3 &m) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double tr[3], ti[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double sr = 0, si = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        sr = fma(r.re[3 * i + k], m.re[3 * k + j], sr);
+        sr = fma(-r.im[3 * i + k], m.im[3 * k + j], sr);
+        si = fma(r.re[3 * i + k], m.im[3 * k + j], si);
+        si = fma(r.im[3 * i + k], m.re[3 * k + j], si);
+      }
+      tr[j] = sr;
+      ti[j] = si;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      r.re[3 * i + j] = tr[j];
+      r.im[3 * i + j] = ti[j];
+    }
+  }
 }
 
 // Product of the links along `n` signed steps (+(d+1) forward, -(d+1)
 // backward) from vertex y, accumulated into acc with weight c.
-template <typename T>
-__device__ void walk_acc(const GFArgs<T> &a, const int *y0, const int *steps, int n, double c,
-                         DM3 &acc) {
+template <int N, typename T>
+__device__ __forceinline__ void walk_acc(const GFArgs<T> &a, const int *y0, const int *steps, double c,
+                                         DM3 &acc) {
   int y[4] = {y0[0], y0[1], y0[2], y0[3]};
-  DM3 r, m, t;
-  for (int k = 0; k < n; ++k) {
+  DM3 r, m;
+#pragma unroll 1
+  for (int k = 0; k < N; ++k) {
     const int s = steps[k], d = (s > 0 ? s : -s) - 1;
     if (s > 0) {
       load_link(a, y, d, false, m);
@@ -97,10 +132,7 @@ __device__ void walk_acc(const GFArgs<T> &a, const int *y0, const int *steps, in
       load_link(a, y, d, true, m);
     }
     if (k == 0) r = m;
-    else {
-      dm_mul(r, m, t);
-      r = t;
-    }
+    else dm_rmul(r, m);
   }
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
@@ -109,22 +141,28 @@ __device__ void walk_acc(const GFArgs<T> &a, const int *y0, const int *steps, in
   }
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs; renumbering them so that
-// XCD x sweeps the x-th contiguous slab of links keeps the paths' link loads
-// (up to 2 sites away) inside that XCD's L2 instead of spreading every
-// lattice region over all eight caches.
+// Grid: x over sites of one parity (blocks renumbered so that XCD x sweeps
+// the x-th contiguous slab, keeping the paths' link loads, up to 2 sites away,
+// in that XCD's L2), y = direction mu, z = parity.  All lanes of a wavefront
+// walk the same loops from consecutive sites, so every link load of a step is
+// coalesced.
 __device__ __forceinline__ int gf_block_slot() {
   const int nb = gridDim.x, b = blockIdx.x;
   return nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
 }
 
-template <typename T> __global__ void __launch_bounds__(256) k_gauge_force(const GFArgs<T> a) {
-  const int l = gf_block_slot() * blockDim.x + threadIdx.x;
-  const int nlink = 8 * a.g.vh;
-  if (l >= nlink) return;
-  const int cb = l / (4 * a.g.vh);
-  const int rem = l - cb * 4 * a.g.vh;
-  const int i = rem >> 2, mu = rem & 3;
+// Occupancy target (waves per SIMD) for the force kernel; unset, the compiler
+// keeps every walk's registers (2 waves per SIMD).
+#ifdef QPHIX_GF_WAVES_PER_EU
+#define QPHIX_GF_OCC __attribute__((amdgpu_waves_per_eu(QPHIX_GF_WAVES_PER_EU)))
+#else
+#define QPHIX_GF_OCC
+#endif
+
+template <typename T> __global__ void __launch_bounds__(256) QPHIX_GF_OCC k_gauge_force(const GFArgs<T> a) {
+  const int i = gf_block_slot() * blockDim.x + threadIdx.x;
+  if (i >= a.g.vh) return;
+  const int mu = blockIdx.y, cb = blockIdx.z;
   int x[4];
   {
     const int xh = i % a.g.nxh;
@@ -141,34 +179,42 @@ template <typename T> __global__ void __launch_bounds__(256) k_gauge_force(const
 #pragma unroll
   for (int e = 0; e < 9; ++e) S.re[e] = S.im[e] = 0;
   const int m = mu + 1;
+  // one loop walk live at a time: unrolling these would interleave walks and
+  // overflow the register file
+#pragma unroll 1
   for (int nu = 0; nu < 4; ++nu) {
     if (nu == mu) continue;
+#pragma unroll 1
     for (int sg = -1; sg <= 1; sg += 2) {
       const int n = sg * (nu + 1);
       if (a.c[0] != 0) {
         const int p[3] = {n, -m, -n};
-        walk_acc(a, y, p, 3, a.c[0], S);
+        walk_acc<3>(a, y, p, a.c[0], S);
       }
       if (a.c[1] != 0) {
         const int ra[5] = {m, n, -m, -m, -n};
         const int rb[5] = {n, -m, -m, -n, m};
         const int rc[5] = {n, n, -m, -n, -n};
-        walk_acc(a, y, ra, 5, a.c[1], S);
-        walk_acc(a, y, rb, 5, a.c[1], S);
-        walk_acc(a, y, rc, 5, a.c[1], S);
+        walk_acc<5>(a, y, ra, a.c[1], S);
+        walk_acc<5>(a, y, rb, a.c[1], S);
+        walk_acc<5>(a, y, rc, a.c[1], S);
       }
     }
   }
   if (a.c[2] != 0) {
+#pragma unroll 1
     for (int da = 0; da < 4; ++da) {
       if (da == mu) continue;
+#pragma unroll 1
       for (int db = 0; db < 4; ++db) {
         if (db == mu || db == da) continue;
+#pragma unroll 1
         for (int sa = -1; sa <= 1; sa += 2)
+#pragma unroll 1
           for (int sb = -1; sb <= 1; sb += 2) {
             const int A = sa * (da + 1), B = sb * (db + 1);
             const int p[5] = {A, B, -m, -A, -B};
-            walk_acc(a, y, p, 5, a.c[2], S);
+            walk_acc<5>(a, y, p, a.c[2], S);
           }
       }
     }
@@ -208,8 +254,7 @@ static void gauge_force(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_ga
   a.c[1] = c->rectangle;
   a.c[2] = c->parallelogram;
   a.eps = eps;
-  const int nlink = 8 * gg.vh;
-  hipLaunchKernelGGL(k_gauge_force<T>, dim3((nlink + 255) / 256), dim3(256), 0, ctx().stream, a);
+  hipLaunchKernelGGL(k_gauge_force<T>, dim3((gg.vh + 255) / 256, 4, 2), dim3(256), 0, ctx().stream, a);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
   info->status = QPHIX_SUCCESS;

@@ -23,6 +23,12 @@ def test_gauge_force_matches_oracle(coef):
     raw_u, raw_m = F.to_raw(U), F.to_raw(m0)
     G = A.Gauge(s, raw_u)
     Fm = A.Force(s, raw_m)
+    # the device layout is complex-major; extract returns the raw array exactly
+    np.testing.assert_array_equal(G.extract(), raw_u)
+    odd = np.zeros_like(raw_u)
+    s.fn("D", "extract_G_to_raw")(A.ptr(odd), G.h, A.ODD)
+    np.testing.assert_array_equal(odd[36 * lat.V:], raw_u[36 * lat.V:])
+    assert not odd[:36 * lat.V].any()
     # C13 round trip; Im A22 is rebuilt as -Im A00 - Im A11 (rounding only)
     np.testing.assert_allclose(Fm.extract(), raw_m, rtol=0, atol=1e-14)
     info = A.gauge_force(G, Fm, coef, 0.02)
