@@ -6,8 +6,41 @@ namespace qphix {
 
 constexpr int DSLASH_BLOCK = 256;
 
+// Site traversal order.  In site order a site's t +- 1, 3 neighbours are
+// read 1-3 t-slices earlier or later (3 MB per slice for 8 interleaved FP32
+// sources at 32^3x64), beyond a 4 MB XCD L2, so they are re-fetched.  With
+// `on`, XCD x's slab of t-slices is walked in (y, z) patches of py x pz rows:
+// all t of the slab for one patch, then the next patch, so the neighbours a
+// patch needs (its +-3 halo over 7 t-slices) stay in L2.  A pure permutation
+// of which lane computes which site: results are unchanged.
+struct SiteOrder {
+  int on, py, pz, tslab;  // tslab = nt / N_XCD
+};
+
+__device__ __forceinline__ int ordered_site(const Geo &g, const SiteOrder &o, int L) {
+  if (!o.on) return L;
+  const int per_xcd = g.nxyz * o.tslab;
+  const int x = L / per_xcd;
+  int r = L - x * per_xcd;
+  const int patch = g.nxh * o.py * o.pz;  // sites of one patch in one t-slice
+  const int column = patch * o.tslab;
+  const int pidx = r / column;
+  r -= pidx * column;
+  const int tt = r / patch;
+  r -= tt * patch;
+  const int xh = r % g.nxh;
+  r /= g.nxh;
+  const int yy = r % o.py, zz = r / o.py;
+  const int npy = g.ny / o.py;
+  const int y = (pidx % npy) * o.py + yy, z = (pidx / npy) * o.pz + zz;
+  const int t = x * o.tslab + tt;
+  return xh + g.nxh * (y + g.ny * (z + g.nz * t));
+}
+
+
+
 template <typename TL, typename SI, typename SO> struct DslashKArgs {
-  const typename vec16<TL>::type *fat;  // output parity's links, chunk-major
+  const typename vec16<TL>::type *fat;  // output parity's links, tile-major
   const typename vec16<TL>::type *lng;
   const uint8_t *lsign;                 // 12-recon signs (bit 2mu+1 fwd, 2mu back)
   const SI *in;
@@ -20,6 +53,7 @@ template <typename TL, typename SI, typename SO> struct DslashKArgs {
   SO *r, *p;
   ReduceCtl rc;
   const int *gate;
+  SiteOrder order;  // OPT_ORDER
 };
 
 // Kernel variants (compile-time bit mask):
@@ -28,7 +62,9 @@ template <typename TL, typename SI, typename SO> struct DslashKArgs {
 //            sites, keeping its neighbour gathers inside its own L2
 //   OPT_NTL  links are loaded non-temporally (streamed, not cached)
 //   OPT_NTS  the output is stored non-temporally
-enum { OPT_XCD = 1, OPT_NTL = 2, OPT_NTS = 4 };
+//   OPT_ORDER sites are walked in (y, z) patches through each XCD's t-slab
+//            (ordered_site; needs OPT_XCD)
+enum { OPT_XCD = 1, OPT_NTL = 2, OPT_NTS = 4, OPT_ORDER = 8 };
 constexpr int N_XCD = 8;
 
 __device__ __forceinline__ int block_slot(int opt) {
@@ -72,8 +108,9 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash(const DslashKArgs<TL, S
   constexpr int NCH_F = 18 / CPV, NCH_L = NL / CPV;
   constexpr bool NTL = (OPT & OPT_NTL) != 0;
   const int vh = a.g.vh;
-  const int i = block_slot(OPT) * blockDim.x + threadIdx.x;
-  const bool valid = i < vh;
+  const int L = block_slot(OPT) * blockDim.x + threadIdx.x;
+  const bool valid = L < vh;
+  const int i = (OPT & OPT_ORDER) && valid ? ordered_site(a.g, a.order, L) : L;
   cplx<TL> acc[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) acc[c] = cplx<TL>{0, 0};

@@ -31,6 +31,7 @@ template <typename T> struct MrhsArgsT {
   double *partials;  // EPI_PKP: [NR][gridDim.x]
   const SolverState *states;  // per-RHS gates (states[r].active)
   int nlog;          // logical 256-item blocks (vh * NR / K / 256, rounded up)
+  SiteOrder order;   // traversal order of the sites
 };
 
 template <int NR> __device__ __forceinline__ unsigned rhs_mask(const SolverState *st) {
@@ -137,9 +138,10 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs(const MrhsArgsT<T>
     const int blk = wk.at(step);
     if (blk < 0) break;
     const int t = blk * DSLASH_BLOCK + threadIdx.x;
-    const int i = t / G, r0 = (t % G) * K;
+    const int L = t / G, r0 = (t % G) * K;
     const unsigned m = (mask >> r0) & ((1u << K) - 1u);
-    if (i >= vh || m == 0) continue;
+    if (L >= vh || m == 0) continue;
+    const int i = ordered_site(a.g, a.order, L);
     cplx<T> acc[K][3];
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -154,7 +156,8 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs(const MrhsArgsT<T>
     // With K > 1 an unrolled direction loop lets the scheduler hoist all 16 x K
     // gathers and overflow the register file; one direction at a time keeps
     // 2 link pairs + 4 K gathers in flight.
-#pragma unroll(K == 1 ? 4 : 1)
+    constexpr int MU_UNROLL = K == 1 ? 4 : 1;
+#pragma unroll MU_UNROLL
     for (int mu = 0; mu < 4; ++mu) {
       apply_pair_k<T, 18, NR, K>(a.fat + (tile * 4 * NCH_F + mu * NCH_F) * LINK_TILE, ii, a.in, n.f1[mu],
                                  n.b1[mu], r0, 0, 0, acc);
