@@ -6,7 +6,7 @@ namespace qphix {
 
 constexpr int DSLASH_BLOCK = 256;
 
-// Site traversal order.  In site order a site's t +- 1, 3 neighbours are
+// Site traversal order (multi-RHS kernels).  In site order a site's t +- 1, 3 neighbours are
 // read 1-3 t-slices earlier or later (3 MB per slice for 8 interleaved FP32
 // sources at 32^3x64), beyond a 4 MB XCD L2, so they are re-fetched.  With
 // `on`, XCD x's slab of t-slices is walked in (y, z) patches of py x pz rows:
@@ -53,7 +53,6 @@ template <typename TL, typename SI, typename SO> struct DslashKArgs {
   SO *r, *p;
   ReduceCtl rc;
   const int *gate;
-  SiteOrder order;  // OPT_ORDER
 };
 
 // Kernel variants (compile-time bit mask):
@@ -62,9 +61,7 @@ template <typename TL, typename SI, typename SO> struct DslashKArgs {
 //            sites, keeping its neighbour gathers inside its own L2
 //   OPT_NTL  links are loaded non-temporally (streamed, not cached)
 //   OPT_NTS  the output is stored non-temporally
-//   OPT_ORDER sites are walked in (y, z) patches through each XCD's t-slab
-//            (ordered_site; needs OPT_XCD)
-enum { OPT_XCD = 1, OPT_NTL = 2, OPT_NTS = 4, OPT_ORDER = 8 };
+enum { OPT_XCD = 1, OPT_NTL = 2, OPT_NTS = 4 };
 constexpr int N_XCD = 8;
 
 __device__ __forceinline__ int block_slot(int opt) {
@@ -108,9 +105,8 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash(const DslashKArgs<TL, S
   constexpr int NCH_F = 18 / CPV, NCH_L = NL / CPV;
   constexpr bool NTL = (OPT & OPT_NTL) != 0;
   const int vh = a.g.vh;
-  const int L = block_slot(OPT) * blockDim.x + threadIdx.x;
-  const bool valid = L < vh;
-  const int i = (OPT & OPT_ORDER) && valid ? ordered_site(a.g, a.order, L) : L;
+  const int i = block_slot(OPT) * blockDim.x + threadIdx.x;
+  const bool valid = i < vh;
   cplx<TL> acc[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) acc[c] = cplx<TL>{0, 0};
