@@ -40,6 +40,14 @@ int QPHIX_MI355X_device_info(int *n_cu, double *hbm_gib);
  * +-conj(row0 x row1) in its third row, else 18), 18 = always 18.
  * Applies to links created afterwards. */
 void QPHIX_MI355X_set_long_recon(int mode);
+/* Gauge-field copy read by the gauge force: 0 = automatic (rows 0-1 only,
+ * row 2 rebuilt as conj(row0 x row1), when every link is SU(3) to rounding;
+ * else all 3 rows), 18 = always 3 rows.  Applies to fields created afterwards;
+ * extract_G always returns the links as given. */
+void QPHIX_MI355X_set_gauge_recon(int mode);
+/* Rows the gauge force reads per link of G: 12 or 18 reals. */
+int QPHIX_D3_gauge_links_recon(QPHIX_D3_GaugeField *G);
+int QPHIX_F3_gauge_links_recon(QPHIX_F3_GaugeField *G);
 /* on != 0 (default): the solvers (CG, multi-shift, mixed and batched) capture
  * their batch of iterations once per solve as a hipGraph and replay it;
  * on == 0: plain launches (the same kernels in the same order, hence

@@ -144,12 +144,19 @@ template <typename T> struct VectorT {
 // Gauge links per parity, complex-major: element e (row-major 3x3) of the link
 // in direction mu at site i is the complex number at index (mu*9 + e)*vh + i,
 // so lanes at consecutive sites read consecutive complexes.
+// u12 (null unless every link is SU(3) to rounding) holds rows 0-1 only, the
+// same way with 6 elements per link (gauge_elem_n with ne = 6); the force
+// kernel rebuilds row 2 as conj(r0 x r1).
 template <typename T> struct GaugeT {
   T *u[2] = {nullptr, nullptr};
+  T *u12[2] = {nullptr, nullptr};
   int parity = 0;
 };
+__host__ __device__ __forceinline__ size_t gauge_elem_n(int mu, int e, int i, int vh, int ne) {
+  return (size_t)(mu * ne + e) * vh + i;
+}
 __host__ __device__ __forceinline__ size_t gauge_elem(int mu, int e, int i, int vh) {
-  return (size_t)(mu * 9 + e) * vh + i;
+  return gauge_elem_n(mu, e, i, vh, 9);
 }
 template <typename T> struct ForceT {
   T *h[2] = {nullptr, nullptr};

@@ -425,6 +425,44 @@ template <typename T> __global__ void k_gauge_out(T *raw, const T *src, int vh) 
     }
 }
 
+// Per link of one parity (complex-major): |r2 - conj(r0 x r1)|, the largest
+// over the links into *maxbits (as the bits of a non-negative double).
+template <typename T> __global__ void k_gauge_su3_check(const T *u, int vh, unsigned long long *maxbits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double worst = 0;
+  if (i < vh) {
+    for (int mu = 0; mu < 4; ++mu) {
+      double re[9], im[9];
+      for (int e = 0; e < 9; ++e) {
+        re[e] = u[2 * gauge_elem(mu, e, i, vh)];
+        im[e] = u[2 * gauge_elem(mu, e, i, vh) + 1];
+      }
+      for (int c = 0; c < 3; ++c) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        const double xr = re[c1] * re[3 + c2] - im[c1] * im[3 + c2] - (re[c2] * re[3 + c1] - im[c2] * im[3 + c1]);
+        const double xi = re[c1] * im[3 + c2] + im[c1] * re[3 + c2] - (re[c2] * im[3 + c1] + im[c2] * re[3 + c1]);
+        worst = fmax(worst, fabs(re[6 + c] - xr));
+        worst = fmax(worst, fabs(im[6 + c] + xi));
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) worst = fmax(worst, __shfl_down(worst, off, 64));
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(maxbits, worst);
+}
+
+// Rows 0-1 of every link, complex-major with 6 elements per link.
+template <typename T> __global__ void k_gauge_rows(T *dst, const T *u, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  for (int mu = 0; mu < 4; ++mu)
+    for (int e = 0; e < 6; ++e) {
+      dst[2 * gauge_elem_n(mu, e, i, vh, 6)] = u[2 * gauge_elem(mu, e, i, vh)];
+      dst[2 * gauge_elem_n(mu, e, i, vh, 6) + 1] = u[2 * gauge_elem(mu, e, i, vh) + 1];
+    }
+}
+
+static int g_gauge_recon_mode = 0;  // 18: never keep the two-row copy
+
 // ------------------------------------------------------------------ G, F
 // Gauge field: all links are always loaded (the reference reads both raw
 // blocks whatever the parity, qphix_d3.cpp:198-200); EVENODD is recorded as
@@ -448,6 +486,38 @@ template <typename T> static typename Handles<T>::G *create_G(const T *raw, int 
     sync();
   }
   dev_free(stage);
+  // Two-row copy for the force kernel when every link is SU(3) to rounding
+  // (the exact links stay in u for extract_G).
+  bool su3 = g_gauge_recon_mode != 18;
+  unsigned long long *dmax = su3 ? (unsigned long long *)dev_alloc(sizeof(unsigned long long)) : nullptr;
+  if (!dmax) su3 = false;
+  for (int cb = 0; cb < 2 && su3; ++cb) {
+    HIP_CHECK(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), ctx().stream));
+    hipLaunchKernelGGL(k_gauge_su3_check<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, G->u[cb], vh, dmax);
+    HIP_CHECK(hipGetLastError());
+    unsigned long long bits = 0;
+    d2h(&bits, dmax, sizeof(bits));
+    sync();
+    double worst;
+    std::memcpy(&worst, &bits, sizeof(worst));
+    if (!(worst <= recon_tol<T>())) su3 = false;
+  }
+  dev_free(dmax);
+  if (su3) {
+    for (int cb = 0; cb < 2; ++cb) G->u12[cb] = (T *)dev_alloc(sizeof(T) * 48 * (size_t)vh);
+    if (!G->u12[0] || !G->u12[1]) {  // no room: the force kernel reads the full links
+      dev_free(G->u12[0]);
+      dev_free(G->u12[1]);
+      G->u12[0] = G->u12[1] = nullptr;
+    } else {
+      for (int cb = 0; cb < 2; ++cb) {
+        hipLaunchKernelGGL(k_gauge_rows<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, G->u12[cb],
+                           G->u[cb], vh);
+        HIP_CHECK(hipGetLastError());
+      }
+    }
+  }
+  sync();
   return G;
 }
 
@@ -473,6 +543,8 @@ template <typename T> static void destroy_G(typename Handles<T>::G *G) {
   if (ctx().initialised) HIP_CHECK(hipStreamSynchronize(ctx().stream));
   dev_free(G->u[0]);
   dev_free(G->u[1]);
+  dev_free(G->u12[0]);
+  dev_free(G->u12[1]);
   delete G;
 }
 
@@ -560,6 +632,9 @@ extern "C" {
     extract_G<T>(raw, G, eo);                                                                     \
   }                                                                                               \
   void QPHIX_##P##3_destroy_G(QPHIX_##P##3_GaugeField *G) { destroy_G<T>(G); }                    \
+  int QPHIX_##P##3_gauge_links_recon(QPHIX_##P##3_GaugeField *G) {                                \
+    return G && G->u12[0] ? 12 : 18;                                                              \
+  }                                                                                               \
   QPHIX_##P##3_Force *QPHIX_##P##3_create_F_from_raw(T *raw, QPHIX_evenodd_t evenodd) {           \
     return create_F<T>(raw, evenodd);                                                             \
   }                                                                                               \
@@ -572,6 +647,7 @@ QPHIX_FIELD_API(D, double)
 QPHIX_FIELD_API(F, float)
 
 void QPHIX_MI355X_set_long_recon(int mode) { g_recon_mode = mode == 18 ? 18 : 0; }
+void QPHIX_MI355X_set_gauge_recon(int mode) { g_gauge_recon_mode = mode == 18 ? 18 : 0; }
 
 void QPHIX_D3_asqtad_links_backcheck(QPHIX_D3_FermionLinksAsqtad *L, double out[4]) {
   backcheck<double>(L, out);

@@ -22,6 +22,8 @@
 #include "handles.h"
 
 #include <chrono>
+#include <cstring>
+#include <initializer_list>
 #include <type_traits>
 
 namespace qphix {
@@ -31,17 +33,37 @@ struct GFGeo {
   int nxh, vh;
 };
 
+// Steps of the 48 loop walks through one link: 6 plaquettes x 3 steps,
+// 18 rectangles x 5, 24 parallelograms x 5.
+constexpr int GF_MAX_STEPS = 6 * 3 + 18 * 5 + 24 * 5;
+
 template <typename T> struct GFArgs {
-  const T *u[2];
+  const T *u[2];    // all 3 rows (the link of the force's own site)
+  const T *w0;      // links the walks read, parity 0: u[0], or the two-row copy
+  ptrdiff_t pdiff;  // parity 1 - parity 0 of those, in units of 2 T
   T *h[2];
   GFGeo g;
   double c[3];
   double eps;
+  int nsteps;       // steps per direction (walks of zero coefficient left out)
+  // per direction mu, one byte per step (see step_link), four to a word: the
+  // kernel reads them with uniform indices, as scalar dword loads
+  // (one padding step after the last: the pipeline's final prefetch loads it)
+  uint32_t tab[4][(GF_MAX_STEPS + 1 + 3) / 4];
 };
+
+template <typename T> __device__ __forceinline__ unsigned gf_step(const GFArgs<T> &a, int mu, int k) {
+  return (a.tab[mu][k >> 2] >> (8 * (k & 3))) & 0xffu;
+}
 
 struct DM3 {
   double re[9], im[9];
 };
+
+// Work-items per block of the force kernel.  Each keeps its staple sum
+// (18 doubles) in LDS, column-wise ([element][lane]), instead of registers: it
+// is touched once per loop walk, and the 36 registers it frees raise occupancy.
+constexpr int GF_BLOCK = 256;
 
 __device__ __forceinline__ void dm_mul(const DM3 &a, const DM3 &b, DM3 &r) {
 #pragma unroll
@@ -61,8 +83,14 @@ __device__ __forceinline__ void dm_mul(const DM3 &a, const DM3 &b, DM3 &r) {
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void load_link(const GFArgs<T> &a, const int *y, int d, bool dag, DM3 &m) {
+// Link U_d(y) (dag = false), or the transpose of U_d(y) (dag = true: the
+// elements of U_d(y)^dag with their imaginary parts not yet negated; products
+// apply that sign, so nothing consumes the loaded values before they are
+// multiplied and the loads stay in flight).  NE = 6: the walks' links hold
+// rows 0-1 only; the elements of row 2 of U (row 2 of m, or column 2 when
+// dag) are left for recon_row2 at the time of use.
+template <int NE, typename T>
+__device__ __forceinline__ void load_link_t(const GFArgs<T> &a, const int *y, int d, bool dag, DM3 &m) {
   int c[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -75,32 +103,59 @@ __device__ __forceinline__ void load_link(const GFArgs<T> &a, const int *y, int 
   // complex-major storage: one 2 x T load per element, coalesced across lanes
   // that walk the same loop from consecutive sites
   using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
-  const T2 *s = reinterpret_cast<const T2 *>(a.u[p]);
+  // base of parity p as address arithmetic: a.u[p] (or a select the compiler
+  // folds back into it) becomes a per-lane load from the kernel arguments
+  const T2 *s = reinterpret_cast<const T2 *>(a.w0) + (ptrdiff_t)p * a.pdiff;
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) {
+      if (NE == 6 && (dag ? cc : r) == 2) continue;
       const int src = dag ? 3 * cc + r : 3 * r + cc;
-      const T2 v = s[gauge_elem(d, src, i, a.g.vh)];
+      const T2 v = s[gauge_elem_n(d, src, i, a.g.vh, NE)];
       m.re[3 * r + cc] = (double)v.x;
-      m.im[3 * r + cc] = dag ? -(double)v.y : (double)v.y;
+      m.im[3 * r + cc] = (double)v.y;
     }
 }
 
-// r <- r * m, one row at a time (one row of temporaries instead of a third
-// matrix: the walk keeps two 3x3 FP64 matrices live, which sets occupancy).
-__device__ __forceinline__ void dm_rmul(DM3 &r, const DM3 &m) {
+// Row 2 of U from rows 0-1: U2 = conj(U0 x U1), on m as load_link_t<6>
+// leaves it (raw elements: m = U, or m = U^T when dag).
+template <bool DAG> __device__ __forceinline__ void recon_row2_t(DM3 &m) {
+  // element (row a, column b) of U in m (compile-time indices keep m in registers)
+  auto at = [](int a, int b) { return DAG ? 3 * b + a : 3 * a + b; };
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+    const double ar = m.re[at(0, c1)], ai = m.im[at(0, c1)], br = m.re[at(1, c2)], bi = m.im[at(1, c2)];
+    const double er = m.re[at(0, c2)], ei = m.im[at(0, c2)], fr = m.re[at(1, c1)], fi = m.im[at(1, c1)];
+    m.re[at(2, c)] = (ar * br - ai * bi) - (er * fr - ei * fi);
+    m.im[at(2, c)] = -((ar * bi + ai * br) - (er * fi + ei * fr));
+  }
+}
+__device__ __forceinline__ void recon_row2(DM3 &m, bool dag) {
+  if (dag) recon_row2_t<true>(m);
+  else recon_row2_t<false>(m);
+}
+
+// r <- r * m', m' = m with its imaginary parts times sg (+-1), one row at a
+// time (one row of temporaries instead of a third matrix).
+__device__ __forceinline__ void dm_rmul(DM3 &r, const DM3 &m, double sg) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    double tr[3], ti[3];
+    double tr[3], ti[3], pre[3], pim[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      pre[k] = sg * r.re[3 * i + k];
+      pim[k] = -sg * r.im[3 * i + k];
+    }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       double sr = 0, si = 0;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         sr = fma(r.re[3 * i + k], m.re[3 * k + j], sr);
-        sr = fma(-r.im[3 * i + k], m.im[3 * k + j], sr);
-        si = fma(r.re[3 * i + k], m.im[3 * k + j], si);
+        sr = fma(pim[k], m.im[3 * k + j], sr);
+        si = fma(pre[k], m.im[3 * k + j], si);
         si = fma(r.im[3 * i + k], m.re[3 * k + j], si);
       }
       tr[j] = sr;
@@ -114,30 +169,36 @@ __device__ __forceinline__ void dm_rmul(DM3 &r, const DM3 &m) {
   }
 }
 
-// Product of the links along `n` signed steps (+(d+1) forward, -(d+1)
-// backward) from vertex y, accumulated into acc with weight c.
-template <int N, typename T>
-__device__ __forceinline__ void walk_acc(const GFArgs<T> &a, const int *y0, const int *steps, double c,
-                                         DM3 &acc) {
-  int y[4] = {y0[0], y0[1], y0[2], y0[3]};
-  DM3 r, m;
-#pragma unroll 1
-  for (int k = 0; k < N; ++k) {
-    const int s = steps[k], d = (s > 0 ? s : -s) - 1;
-    if (s > 0) {
-      load_link(a, y, d, false, m);
-      y[d] += 1;
-    } else {
-      y[d] -= 1;
-      load_link(a, y, d, true, m);
-    }
-    if (k == 0) r = m;
-    else dm_rmul(r, m);
+// One step of a loop walk, as a byte of the per-direction step table:
+// bits 0-1 direction d, bit 2 backward, bit 3 first step of a walk (the walk
+// starts at x + mu), bit 4 last step (the walk's product is added to the
+// staple sum), bits 5-6 index of the walk's coefficient.
+enum : unsigned { GF_BACK = 4, GF_FIRST = 8, GF_LAST = 16 };
+
+// Link of one step from vertex y, as load_link_t leaves it (a backward step's
+// imaginary parts still to be negated); y moves to the step's end.
+template <int NE, typename T>
+__device__ __forceinline__ void step_link(const GFArgs<T> &a, int *y, unsigned e, DM3 &m) {
+  const int d = e & 3;
+  if (e & GF_BACK) {
+    y[d] -= 1;
+    load_link_t<NE>(a, y, d, true, m);
+  } else {
+    load_link_t<NE>(a, y, d, false, m);
+    y[d] += 1;
   }
+}
+
+// The force's own link U_mu(x), all 3 rows, x at index i of parity cb.
+template <typename T>
+__device__ __forceinline__ void own_link(const GFArgs<T> &a, int cb, int mu, int i, DM3 &m) {
+  using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  const T2 *s = reinterpret_cast<const T2 *>(cb ? a.u[1] : a.u[0]);  // cb is uniform
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
-    acc.re[e] = fma(c, r.re[e], acc.re[e]);
-    acc.im[e] = fma(c, r.im[e], acc.im[e]);
+    const T2 v = s[gauge_elem(mu, e, i, a.g.vh)];
+    m.re[e] = (double)v.x;
+    m.im[e] = (double)v.y;
   }
 }
 
@@ -159,7 +220,8 @@ __device__ __forceinline__ int gf_block_slot() {
 #define QPHIX_GF_OCC
 #endif
 
-template <typename T> __global__ void __launch_bounds__(256) QPHIX_GF_OCC k_gauge_force(const GFArgs<T> a) {
+template <typename T, int NE>
+__global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFArgs<T> a) {
   const int i = gf_block_slot() * blockDim.x + threadIdx.x;
   if (i >= a.g.vh) return;
   const int mu = blockIdx.y, cb = blockIdx.z;
@@ -175,53 +237,72 @@ template <typename T> __global__ void __launch_bounds__(256) QPHIX_GF_OCC k_gaug
   }
   int y[4] = {x[0], x[1], x[2], x[3]};
   y[mu] += 1;  // every rest path starts at x + mu
-  DM3 S;
+  __shared__ double sacc[18 * GF_BLOCK];
+  double *S = sacc + threadIdx.x;
 #pragma unroll
-  for (int e = 0; e < 9; ++e) S.re[e] = S.im[e] = 0;
-  const int m = mu + 1;
-  // one loop walk live at a time: unrolling these would interleave walks and
-  // overflow the register file
-#pragma unroll 1
-  for (int nu = 0; nu < 4; ++nu) {
-    if (nu == mu) continue;
-#pragma unroll 1
-    for (int sg = -1; sg <= 1; sg += 2) {
-      const int n = sg * (nu + 1);
-      if (a.c[0] != 0) {
-        const int p[3] = {n, -m, -n};
-        walk_acc<3>(a, y, p, a.c[0], S);
+  for (int e = 0; e < 18; ++e) S[e * GF_BLOCK] = 0;
+  // All 48 loop walks through (x, mu) as one software-pipelined stream of
+  // steps: the link of step k + 1 (of this walk or the next) is requested
+  // before the product with step k's link, so each load's latency overlaps a
+  // 3x3 product.  Three matrices are live (product, current link, next link).
+  // The step bytes are uniform over the grid's y row, so the branches on them
+  // are uniform.
+  const int n = a.nsteps;
+  // one step: request the link of step k + 1 into `next`, then apply step k's
+  // link `cur` (its table byte e); returns step k + 1's byte
+  // (always a load, also after the last step: a conditional one leaves the
+  // number of loads in flight unknown at the join, and the compiler then
+  // waits for all of them)
+  auto step = [&](int k, unsigned e, int *w, DM3 &r, DM3 &cur, DM3 &next) -> unsigned {
+    const unsigned en = gf_step(a, mu, k + 1);
+    if (en & GF_FIRST) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) w[q] = y[q];
+    }
+    step_link<NE>(a, w, en, next);
+    if constexpr (NE == 6) recon_row2(cur, (e & GF_BACK) != 0);
+    const double sg = (e & GF_BACK) ? -1.0 : 1.0;
+    if (e & GF_FIRST) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        r.re[q] = cur.re[q];
+        r.im[q] = sg * cur.im[q];
       }
-      if (a.c[1] != 0) {
-        const int ra[5] = {m, n, -m, -m, -n};
-        const int rb[5] = {n, -m, -m, -n, m};
-        const int rc[5] = {n, n, -m, -n, -n};
-        walk_acc<5>(a, y, ra, a.c[1], S);
-        walk_acc<5>(a, y, rb, a.c[1], S);
-        walk_acc<5>(a, y, rc, a.c[1], S);
+    } else {
+      dm_rmul(r, cur, sg);
+    }
+    if (e & GF_LAST) {
+      const double c = a.c[(e >> 5) & 3];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        S[(2 * q) * GF_BLOCK] += c * r.re[q];
+        S[(2 * q + 1) * GF_BLOCK] += c * r.im[q];
       }
     }
-  }
-  if (a.c[2] != 0) {
+    return en;
+  };
+  if (n > 0) {
+    int w[4] = {y[0], y[1], y[2], y[3]};
+    DM3 r, b0, b1;
+    unsigned e = gf_step(a, mu, 0);
+    step_link<NE>(a, w, e, b0);
+    // two steps per trip with the link buffers swapping roles, so no buffer
+    // is copied (a copy would wait for the link in flight)
+    // (n is even: every walk type contributes an even number of steps)
 #pragma unroll 1
-    for (int da = 0; da < 4; ++da) {
-      if (da == mu) continue;
-#pragma unroll 1
-      for (int db = 0; db < 4; ++db) {
-        if (db == mu || db == da) continue;
-#pragma unroll 1
-        for (int sa = -1; sa <= 1; sa += 2)
-#pragma unroll 1
-          for (int sb = -1; sb <= 1; sb += 2) {
-            const int A = sa * (da + 1), B = sb * (db + 1);
-            const int p[5] = {A, B, -m, -A, -B};
-            walk_acc<5>(a, y, p, a.c[2], S);
-          }
-      }
+    for (int k = 0; k < n; k += 2) {
+      e = step(k, e, w, r, b0, b1);
+      e = step(k + 1, e, w, r, b1, b0);
     }
   }
-  DM3 U, W;
-  load_link(a, x, mu, false, U);
-  dm_mul(U, S, W);
+  DM3 U, W, Ssum;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    Ssum.re[e] = S[(2 * e) * GF_BLOCK];
+    Ssum.im[e] = S[(2 * e + 1) * GF_BLOCK];
+  }
+  own_link(a, cb, mu, i, U);
+  dm_mul(U, Ssum, W);
   // H <- H + i eps TA(W)  (momentum stored as H = -iA, 8 reals)
   const double imtr = (W.im[0] + W.im[4] + W.im[8]) / 3.0;
   T *h = a.h[cb] + 32 * (size_t)i + 8 * mu;
@@ -238,6 +319,68 @@ template <typename T> __global__ void __launch_bounds__(256) QPHIX_GF_OCC k_gaug
   h[7] = (T)((double)h[7] - a.eps * (W.im[4] - imtr));
 }
 
+// Step tables of the loop walks through (x, mu), for each mu.  A walk is a
+// list of signed steps (+(d+1) forward along d, -(d+1) backward) from x + mu
+// back to x; for the link's own direction m = mu + 1 and another direction n:
+//   plaquette   n, -m, -n
+//   rectangles  m, n, -m, -m, -n  /  n, -m, -m, -n, m  /  n, n, -m, -n, -n
+// for n = +-(nu+1), and for two other directions A, B (ordered, signed):
+//   parallelogram  A, B, -m, -A, -B.
+// Walks whose coefficient is zero are left out.
+template <typename T> static bool gf_step_tables(GFArgs<T> &a) {
+  std::memset(a.tab, 0, sizeof(a.tab));
+  int n = 0;
+  for (int mu = 0; mu < 4; ++mu) {
+    int k = 0;
+    auto walk = [&](std::initializer_list<int> steps, unsigned ci) {
+      const int len = (int)steps.size();
+      int j = 0;
+      for (int st : steps) {
+        unsigned e = (unsigned)((st > 0 ? st : -st) - 1) | (ci << 5);
+        if (st < 0) e |= GF_BACK;
+        if (j == 0) e |= GF_FIRST;
+        if (j == len - 1) e |= GF_LAST;
+        a.tab[mu][k >> 2] |= e << (8 * (k & 3));
+        ++k;
+        ++j;
+      }
+    };
+    const int m = mu + 1;
+    for (int nu = 0; nu < 4; ++nu) {
+      if (nu == mu) continue;
+      for (int sg = -1; sg <= 1; sg += 2) {
+        const int d = sg * (nu + 1);
+        if (a.c[0] != 0) walk({d, -m, -d}, 0);
+        if (a.c[1] != 0) {
+          walk({m, d, -m, -m, -d}, 1);
+          walk({d, -m, -m, -d, m}, 1);
+          walk({d, d, -m, -d, -d}, 1);
+        }
+      }
+    }
+    if (a.c[2] != 0) {
+      for (int da = 0; da < 4; ++da) {
+        if (da == mu) continue;
+        for (int db = 0; db < 4; ++db) {
+          if (db == mu || db == da) continue;
+          for (int sa = -1; sa <= 1; sa += 2)
+            for (int sb = -1; sb <= 1; sb += 2) {
+              const int A = sa * (da + 1), B = sb * (db + 1);
+              walk({A, B, -m, -A, -B}, 2);
+            }
+        }
+      }
+    }
+    // padding step: a forward step from x + mu along direction 0, loaded by
+    // the last prefetch and never applied
+    a.tab[mu][k >> 2] |= GF_FIRST << (8 * (k & 3));
+    n = k;
+  }
+  QPHIX_REQUIRE(n % 2 == 0 && n <= GF_MAX_STEPS, "gauge force: bad step table", false);
+  a.nsteps = n;
+  return true;
+}
+
 template <typename T>
 static void gauge_force(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_gauge_coeffs_t *c,
                         double eps) {
@@ -250,11 +393,23 @@ static void gauge_force(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_ga
   a.h[0] = F->h[0];
   a.h[1] = F->h[1];
   a.g = GFGeo{{gg.nx, gg.ny, gg.nz, gg.nt}, gg.nxh, gg.vh};
+  // the walks read the two-row copy when the field has one
+  const bool rows2 = G->u12[0] && G->u12[1];
+  const T *w[2] = {rows2 ? G->u12[0] : G->u[0], rows2 ? G->u12[1] : G->u[1]};
+  a.w0 = w[0];
+  a.pdiff = reinterpret_cast<const char *>(w[1]) - reinterpret_cast<const char *>(w[0]);
+  QPHIX_REQUIRE(a.pdiff % (ptrdiff_t)(2 * sizeof(T)) == 0, "gauge force: misaligned parity arrays");
+  a.pdiff /= (ptrdiff_t)(2 * sizeof(T));
   a.c[0] = c->plaquette;
   a.c[1] = c->rectangle;
   a.c[2] = c->parallelogram;
   a.eps = eps;
-  hipLaunchKernelGGL(k_gauge_force<T>, dim3((gg.vh + 255) / 256, 4, 2), dim3(256), 0, ctx().stream, a);
+  if (!gf_step_tables(a)) return;
+  const dim3 grid((gg.vh + GF_BLOCK - 1) / GF_BLOCK, 4, 2);
+  if (rows2)
+    hipLaunchKernelGGL((k_gauge_force<T, 6>), grid, dim3(GF_BLOCK), 0, ctx().stream, a);
+  else
+    hipLaunchKernelGGL((k_gauge_force<T, 9>), grid, dim3(GF_BLOCK), 0, ctx().stream, a);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
   info->status = QPHIX_SUCCESS;

@@ -93,6 +93,7 @@ class Lib:
             _sig(lib, f"QPHIX_{P}3_asqtad_extract_L_to_raw", None, [_dp, _dp, _dp, C.c_int])
             _sig(lib, f"QPHIX_{P}3_asqtad_destroy_L", None, [_dp])
             _sig(lib, f"QPHIX_{P}3_asqtad_links_recon", C.c_int, [_dp])
+            _sig(lib, f"QPHIX_{P}3_gauge_links_recon", C.c_int, [_dp])
             _sig(lib, f"QPHIX_{P}3_create_G_from_raw", _dp, [_dp, C.c_int])
             _sig(lib, f"QPHIX_{P}3_extract_G_to_raw", None, [_dp, _dp, C.c_int])
             _sig(lib, f"QPHIX_{P}3_destroy_G", None, [_dp])
@@ -125,6 +126,7 @@ class Lib:
         _sig(lib, "QPHIX_MI355X_stream", _dp, [])
         _sig(lib, "QPHIX_MI355X_device_info", C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_double)])
         _sig(lib, "QPHIX_MI355X_set_long_recon", None, [C.c_int])
+        _sig(lib, "QPHIX_MI355X_set_gauge_recon", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_set_graphs", None, [C.c_int])
         _sig(lib, "QPHIX_D3_asqtad_links_backcheck", None, [_dp, C.POINTER(C.c_double)])
         _sig(lib, "QPHIX_MI355X_copy_time", C.c_double, [C.c_size_t, C.c_int])
