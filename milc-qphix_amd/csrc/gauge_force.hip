@@ -22,6 +22,7 @@
 #include "handles.h"
 
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <initializer_list>
 #include <type_traits>
@@ -46,6 +47,7 @@ template <typename T> struct GFArgs {
   double c[3];
   double eps;
   int nsteps;       // steps per direction (walks of zero coefficient left out)
+  int order;        // block order, see gf_block
   // per direction mu, one byte per step (see step_link), four to a word: the
   // kernel reads them with uniform indices, as scalar dword loads
   // (one padding step after the last: the pipeline's final prefetch loads it)
@@ -83,36 +85,62 @@ __device__ __forceinline__ void dm_mul(const DM3 &a, const DM3 &b, DM3 &r) {
     }
 }
 
-// Link U_d(y) (dag = false), or the transpose of U_d(y) (dag = true: the
-// elements of U_d(y)^dag with their imaginary parts not yet negated; products
-// apply that sign, so nothing consumes the loaded values before they are
-// multiplied and the loads stay in flight).  NE = 6: the walks' links hold
-// rows 0-1 only; the elements of row 2 of U (row 2 of m, or column 2 when
-// dag) are left for recon_row2 at the time of use.
-template <int NE, typename T>
-__device__ __forceinline__ void load_link_t(const GFArgs<T> &a, const int *y, int d, bool dag, DM3 &m) {
+// A walk's current vertex: wrapped coordinates and lexicographic index
+// r = x0 + L0 (x1 + L1 (x2 + L2 x3)) (the half-site index is r >> 1, L0 even).
+struct Walker {
   int c[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int v = y[k];
-    v = v < 0 ? v + a.g.L[k] : (v >= a.g.L[k] ? v - a.g.L[k] : v);
-    c[k] = v;
+  int r;
+};
+
+// Moves w one site along direction D, forward (sgn = 1) or backward (-1);
+// only the moved coordinate is wrapped.
+template <int D> __device__ __forceinline__ void walker_move(Walker &w, int sgn, const GFGeo &g) {
+  const int L = g.L[D];
+  const int stride = D == 0 ? 1 : D == 1 ? g.L[0] : D == 2 ? g.L[0] * g.L[1] : g.L[0] * g.L[1] * g.L[2];
+  int v = w.c[D] + sgn, r = w.r + sgn * stride;
+  if (sgn > 0) {  // uniform
+    const bool wrap = v == L;
+    v = wrap ? 0 : v;
+    r = wrap ? r - L * stride : r;
+  } else {
+    const bool wrap = v < 0;
+    v = wrap ? L - 1 : v;
+    r = wrap ? r + L * stride : r;
   }
-  const int p = (c[0] + c[1] + c[2] + c[3]) & 1;
-  const int i = (c[0] >> 1) + a.g.nxh * (c[1] + a.g.L[1] * (c[2] + a.g.L[2] * c[3]));
-  // complex-major storage: one 2 x T load per element, coalesced across lanes
-  // that walk the same loop from consecutive sites
+  w.c[D] = v;
+  w.r = r;
+}
+__device__ __forceinline__ void walker_move(Walker &w, int d, int sgn, const GFGeo &g) {
+  switch (d) {  // d is uniform: one branch, no indexed registers
+    case 0: walker_move<0>(w, sgn, g); break;
+    case 1: walker_move<1>(w, sgn, g); break;
+    case 2: walker_move<2>(w, sgn, g); break;
+    default: walker_move<3>(w, sgn, g); break;
+  }
+}
+
+// Link at the walker's vertex in direction d, of parity p (uniform: every lane
+// of the wavefront is at a vertex of the same parity).  dag = false: U_d;
+// dag = true: the transpose of U_d (the elements of U_d^dag with their
+// imaginary parts not yet negated; the step applies that sign when the link is
+// used, so nothing consumes the loaded values early and the loads stay in
+// flight).  Each element is one load from a uniform base (scalar) plus the
+// lane's site offset.  NE = 6: rows 0-1 only are stored; the elements of row 2
+// of U (row 2 of m, or column 2 when dag) are left for recon_row2.
+template <int NE, typename T>
+__device__ __forceinline__ void load_link_t(const GFArgs<T> &a, const Walker &w, int d, int p, bool dag,
+                                            DM3 &m) {
   using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
-  // base of parity p as address arithmetic: a.u[p] (or a select the compiler
-  // folds back into it) becomes a per-lane load from the kernel arguments
-  const T2 *s = reinterpret_cast<const T2 *>(a.w0) + (ptrdiff_t)p * a.pdiff;
+  const T2 *base = reinterpret_cast<const T2 *>(a.w0) + (ptrdiff_t)p * a.pdiff +
+                   (size_t)(d * NE) * (size_t)a.g.vh;
+  const unsigned i = (unsigned)w.r >> 1;
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) {
       if (NE == 6 && (dag ? cc : r) == 2) continue;
       const int src = dag ? 3 * cc + r : 3 * r + cc;
-      const T2 v = s[gauge_elem_n(d, src, i, a.g.vh, NE)];
+      const T2 v = (base + (size_t)src * (size_t)a.g.vh)[i];
       m.re[3 * r + cc] = (double)v.x;
       m.im[3 * r + cc] = (double)v.y;
     }
@@ -137,25 +165,22 @@ __device__ __forceinline__ void recon_row2(DM3 &m, bool dag) {
   else recon_row2_t<false>(m);
 }
 
-// r <- r * m', m' = m with its imaginary parts times sg (+-1), one row at a
-// time (one row of temporaries instead of a third matrix).
-__device__ __forceinline__ void dm_rmul(DM3 &r, const DM3 &m, double sg) {
+// r <- r * m', m' = m (CONJ false) or m with its imaginary parts negated
+// (CONJ true), one row at a time (one row of temporaries instead of a third
+// matrix).
+template <bool CONJ> __device__ __forceinline__ void dm_rmul_t(DM3 &r, const DM3 &m) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    double tr[3], ti[3], pre[3], pim[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      pre[k] = sg * r.re[3 * i + k];
-      pim[k] = -sg * r.im[3 * i + k];
-    }
+    double tr[3], ti[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       double sr = 0, si = 0;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
+        const double mi = CONJ ? -m.im[3 * k + j] : m.im[3 * k + j];  // folds into the fma signs
         sr = fma(r.re[3 * i + k], m.re[3 * k + j], sr);
-        sr = fma(pim[k], m.im[3 * k + j], sr);
-        si = fma(pre[k], m.im[3 * k + j], si);
+        sr = fma(-r.im[3 * i + k], mi, sr);
+        si = fma(r.re[3 * i + k], mi, si);
         si = fma(r.im[3 * i + k], m.re[3 * k + j], si);
       }
       tr[j] = sr;
@@ -172,20 +197,22 @@ __device__ __forceinline__ void dm_rmul(DM3 &r, const DM3 &m, double sg) {
 // One step of a loop walk, as a byte of the per-direction step table:
 // bits 0-1 direction d, bit 2 backward, bit 3 first step of a walk (the walk
 // starts at x + mu), bit 4 last step (the walk's product is added to the
-// staple sum), bits 5-6 index of the walk's coefficient.
-enum : unsigned { GF_BACK = 4, GF_FIRST = 8, GF_LAST = 16 };
+// staple sum), bits 5-6 index of the walk's coefficient, bit 7 parity of the
+// step's link relative to x (the number of unit moves from x to the link's
+// site, mod 2).
+enum : unsigned { GF_BACK = 4, GF_FIRST = 8, GF_LAST = 16, GF_PODD = 128 };
 
-// Link of one step from vertex y, as load_link_t leaves it (a backward step's
-// imaginary parts still to be negated); y moves to the step's end.
+// Link of one step from the walker's vertex (x's parity cb), as load_link_t
+// leaves it; the walker moves to the step's end.
 template <int NE, typename T>
-__device__ __forceinline__ void step_link(const GFArgs<T> &a, int *y, unsigned e, DM3 &m) {
-  const int d = e & 3;
+__device__ __forceinline__ void step_link(const GFArgs<T> &a, Walker &w, unsigned e, int cb, DM3 &m) {
+  const int d = e & 3, p = cb ^ ((e & GF_PODD) ? 1 : 0);
   if (e & GF_BACK) {
-    y[d] -= 1;
-    load_link_t<NE>(a, y, d, true, m);
+    walker_move(w, d, -1, a.g);
+    load_link_t<NE>(a, w, d, p, true, m);
   } else {
-    load_link_t<NE>(a, y, d, false, m);
-    y[d] += 1;
+    load_link_t<NE>(a, w, d, p, false, m);
+    walker_move(w, d, 1, a.g);
   }
 }
 
@@ -202,14 +229,25 @@ __device__ __forceinline__ void own_link(const GFArgs<T> &a, int cb, int mu, int
   }
 }
 
-// Grid: x over sites of one parity (blocks renumbered so that XCD x sweeps
-// the x-th contiguous slab, keeping the paths' link loads, up to 2 sites away,
-// in that XCD's L2), y = direction mu, z = parity.  All lanes of a wavefront
-// walk the same loops from consecutive sites, so every link load of a step is
-// coalesced.
-__device__ __forceinline__ int gf_block_slot() {
+// Grid: one block per (256 sites of one parity, direction mu, parity).  All
+// lanes of a wavefront walk the same loops from consecutive sites, so every
+// link load of a step is coalesced.  Blocks are renumbered so that XCD x
+// sweeps the x-th contiguous slab of the work, keeping the paths' link loads,
+// up to 2 sites away, in that XCD's L2.  order 0: the slab runs over the sites
+// of one (mu, parity) at a time; order 1: over site blocks, with the 8
+// (mu, parity) blocks of each site block next to each other (they read the
+// same neighbourhood links).
+struct GFBlock {
+  int sb, mu, cb;
+};
+__device__ __forceinline__ GFBlock gf_block(int order, int nbx) {
   const int nb = gridDim.x, b = blockIdx.x;
-  return nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+  if (order == 1) {
+    const int slot = nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+    return {slot >> 3, slot & 3, (slot >> 2) & 1};
+  }
+  const int bx = b % nbx, mu = (b / nbx) & 3, cb = b / (4 * nbx);
+  return {nbx % 8 == 0 ? (bx % 8) * (nbx / 8) + bx / 8 : bx, mu, cb};
 }
 
 // Occupancy target (waves per SIMD) for the force kernel; unset, the compiler
@@ -222,9 +260,10 @@ __device__ __forceinline__ int gf_block_slot() {
 
 template <typename T, int NE>
 __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFArgs<T> a) {
-  const int i = gf_block_slot() * blockDim.x + threadIdx.x;
+  const GFBlock bk = gf_block(a.order, (a.g.vh + GF_BLOCK - 1) / GF_BLOCK);
+  const int i = bk.sb * GF_BLOCK + threadIdx.x;
   if (i >= a.g.vh) return;
-  const int mu = blockIdx.y, cb = blockIdx.z;
+  const int mu = bk.mu, cb = bk.cb;
   int x[4];
   {
     const int xh = i % a.g.nxh;
@@ -235,8 +274,9 @@ __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFA
     x[3] = r / a.g.L[2];
     x[0] = 2 * xh + ((x[1] + x[2] + x[3] + cb) & 1);
   }
-  int y[4] = {x[0], x[1], x[2], x[3]};
-  y[mu] += 1;  // every rest path starts at x + mu
+  // every rest path starts at x + mu
+  Walker y{{x[0], x[1], x[2], x[3]}, x[0] + a.g.L[0] * (x[1] + a.g.L[1] * (x[2] + a.g.L[2] * x[3]))};
+  walker_move(y, mu, 1, a.g);
   __shared__ double sacc[18 * GF_BLOCK];
   double *S = sacc + threadIdx.x;
 #pragma unroll
@@ -253,23 +293,24 @@ __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFA
   // (always a load, also after the last step: a conditional one leaves the
   // number of loads in flight unknown at the join, and the compiler then
   // waits for all of them)
-  auto step = [&](int k, unsigned e, int *w, DM3 &r, DM3 &cur, DM3 &next) -> unsigned {
+  auto step = [&](int k, unsigned e, Walker &w, DM3 &r, DM3 &cur, DM3 &next) -> unsigned {
     const unsigned en = gf_step(a, mu, k + 1);
-    if (en & GF_FIRST) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) w[q] = y[q];
-    }
-    step_link<NE>(a, w, en, next);
+    if (en & GF_FIRST) w = y;
+    step_link<NE>(a, w, en, cb, next);
     if constexpr (NE == 6) recon_row2(cur, (e & GF_BACK) != 0);
-    const double sg = (e & GF_BACK) ? -1.0 : 1.0;
     if (e & GF_FIRST) {
+      const double sg = (e & GF_BACK) ? -1.0 : 1.0;
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
         r.re[q] = cur.re[q];
         r.im[q] = sg * cur.im[q];
       }
     } else {
-      dm_rmul(r, cur, sg);
+      if (e & GF_BACK) {  // uniform branch: conjugate the link in place
+#pragma unroll
+        for (int q = 0; q < 9; ++q) cur.im[q] = -cur.im[q];
+      }
+      dm_rmul_t<false>(r, cur);
     }
     if (e & GF_LAST) {
       const double c = a.c[(e >> 5) & 3];
@@ -282,10 +323,10 @@ __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFA
     return en;
   };
   if (n > 0) {
-    int w[4] = {y[0], y[1], y[2], y[3]};
+    Walker w = y;
     DM3 r, b0, b1;
     unsigned e = gf_step(a, mu, 0);
-    step_link<NE>(a, w, e, b0);
+    step_link<NE>(a, w, e, cb, b0);
     // two steps per trip with the link buffers swapping roles, so no buffer
     // is copied (a copy would wait for the link in flight)
     // (n is even: every walk type contributes an even number of steps)
@@ -338,6 +379,9 @@ template <typename T> static bool gf_step_tables(GFArgs<T> &a) {
       for (int st : steps) {
         unsigned e = (unsigned)((st > 0 ? st : -st) - 1) | (ci << 5);
         if (st < 0) e |= GF_BACK;
+        // unit moves from x to the link's site: 1 (to x + mu) + j, + 1 more
+        // for a backward step (its link sits at the step's end)
+        if ((1 + j + (st < 0 ? 1 : 0)) & 1) e |= GF_PODD;
         if (j == 0) e |= GF_FIRST;
         if (j == len - 1) e |= GF_LAST;
         a.tab[mu][k >> 2] |= e << (8 * (k & 3));
@@ -371,9 +415,9 @@ template <typename T> static bool gf_step_tables(GFArgs<T> &a) {
         }
       }
     }
-    // padding step: a forward step from x + mu along direction 0, loaded by
-    // the last prefetch and never applied
-    a.tab[mu][k >> 2] |= GF_FIRST << (8 * (k & 3));
+    // padding step: a forward step from x + mu (one move from x) along
+    // direction 0, loaded by the last prefetch and never applied
+    a.tab[mu][k >> 2] |= (GF_FIRST | GF_PODD) << (8 * (k & 3));
     n = k;
   }
   QPHIX_REQUIRE(n % 2 == 0 && n <= GF_MAX_STEPS, "gauge force: bad step table", false);
@@ -405,7 +449,12 @@ static void gauge_force(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_ga
   a.c[2] = c->parallelogram;
   a.eps = eps;
   if (!gf_step_tables(a)) return;
-  const dim3 grid((gg.vh + GF_BLOCK - 1) / GF_BLOCK, 4, 2);
+  static const int order = [] {
+    const char *e = std::getenv("QPHIX_GF_ORDER");
+    return e && std::atoi(e) == 0 ? 0 : 1;
+  }();
+  a.order = order;
+  const dim3 grid((unsigned)((gg.vh + GF_BLOCK - 1) / GF_BLOCK * 8));
   if (rows2)
     hipLaunchKernelGGL((k_gauge_force<T, 6>), grid, dim3(GF_BLOCK), 0, ctx().stream, a);
   else
