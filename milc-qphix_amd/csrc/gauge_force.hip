@@ -37,6 +37,9 @@ struct GFGeo {
 // Steps of the 48 loop walks through one link: 6 plaquettes x 3 steps,
 // 18 rectangles x 5, 24 parallelograms x 5.
 constexpr int GF_MAX_STEPS = 6 * 3 + 18 * 5 + 24 * 5;
+// Padding steps = the largest prefetch distance D compiled (distance 2
+// measured the same as 1 at 24^4, with 17 more registers).
+constexpr int GF_PAD = 2;
 
 template <typename T> struct GFArgs {
   const T *u[2];    // all 3 rows (the link of the force's own site)
@@ -50,8 +53,8 @@ template <typename T> struct GFArgs {
   int order;        // block order, see gf_block
   // per direction mu, one byte per step (see step_link), four to a word: the
   // kernel reads them with uniform indices, as scalar dword loads
-  // (one padding step after the last: the pipeline's final prefetch loads it)
-  uint32_t tab[4][(GF_MAX_STEPS + 1 + 3) / 4];
+  // (GF_PAD padding steps after the last: the pipeline's final prefetches load them)
+  uint32_t tab[4][(GF_MAX_STEPS + GF_PAD + 3) / 4];
 };
 
 template <typename T> __device__ __forceinline__ unsigned gf_step(const GFArgs<T> &a, int mu, int k) {
@@ -258,7 +261,7 @@ __device__ __forceinline__ GFBlock gf_block(int order, int nbx) {
 #define QPHIX_GF_OCC
 #endif
 
-template <typename T, int NE>
+template <typename T, int NE, int D>
 __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFArgs<T> a) {
   const GFBlock bk = gf_block(a.order, (a.g.vh + GF_BLOCK - 1) / GF_BLOCK);
   const int i = bk.sb * GF_BLOCK + threadIdx.x;
@@ -282,21 +285,24 @@ __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFA
 #pragma unroll
   for (int e = 0; e < 18; ++e) S[e * GF_BLOCK] = 0;
   // All 48 loop walks through (x, mu) as one software-pipelined stream of
-  // steps: the link of step k + 1 (of this walk or the next) is requested
-  // before the product with step k's link, so each load's latency overlaps a
-  // 3x3 product.  Three matrices are live (product, current link, next link).
-  // The step bytes are uniform over the grid's y row, so the branches on them
-  // are uniform.
+  // steps: the link of step k + D (of this walk or a later one) is requested
+  // before the product with step k's link, so each load's latency overlaps D
+  // 3x3 products.  D + 2 matrices are live (product and D + 1 link buffers).
+  // The step bytes are uniform over the block, so the branches on them are
+  // uniform.
   const int n = a.nsteps;
-  // one step: request the link of step k + 1 into `next`, then apply step k's
-  // link `cur` (its table byte e); returns step k + 1's byte
-  // (always a load, also after the last step: a conditional one leaves the
-  // number of loads in flight unknown at the join, and the compiler then
-  // waits for all of them)
-  auto step = [&](int k, unsigned e, Walker &w, DM3 &r, DM3 &cur, DM3 &next) -> unsigned {
-    const unsigned en = gf_step(a, mu, k + 1);
-    if (en & GF_FIRST) w = y;
-    step_link<NE>(a, w, en, cb, next);
+  // request the link of step k into buf (the walker moves with the requests);
+  // always a load, also for the padding steps after the last: a conditional
+  // one leaves the number of loads in flight unknown, and the compiler then
+  // waits for all of them
+  auto fetch = [&](int k, Walker &w, DM3 &buf) -> unsigned {
+    const unsigned e = gf_step(a, mu, k);
+    if (e & GF_FIRST) w = y;
+    step_link<NE>(a, w, e, cb, buf);
+    return e;
+  };
+  // apply a step's link (table byte e) to the walk's running product r
+  auto apply = [&](unsigned e, DM3 &r, DM3 &cur) {
     if constexpr (NE == 6) recon_row2(cur, (e & GF_BACK) != 0);
     if (e & GF_FIRST) {
       const double sg = (e & GF_BACK) ? -1.0 : 1.0;
@@ -320,20 +326,24 @@ __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFA
         S[(2 * q + 1) * GF_BLOCK] += c * r.im[q];
       }
     }
-    return en;
   };
   if (n > 0) {
     Walker w = y;
-    DM3 r, b0, b1;
-    unsigned e = gf_step(a, mu, 0);
-    step_link<NE>(a, w, e, cb, b0);
-    // two steps per trip with the link buffers swapping roles, so no buffer
-    // is copied (a copy would wait for the link in flight)
-    // (n is even: every walk type contributes an even number of steps)
+    DM3 r, buf[D + 1];
+    unsigned es[D + 1];
+#pragma unroll
+    for (int j = 0; j < D; ++j) es[j] = fetch(j, w, buf[j]);
+    // D + 1 steps per trip, unrolled, with the buffers rotating (compile-time
+    // indices, no copies: a copy would wait for the link in flight); n is a
+    // multiple of 6 (every walk type contributes one)
 #pragma unroll 1
-    for (int k = 0; k < n; k += 2) {
-      e = step(k, e, w, r, b0, b1);
-      e = step(k + 1, e, w, r, b1, b0);
+    for (int k = 0; k < n; k += D + 1) {
+#pragma unroll
+      for (int j = 0; j <= D; ++j) {
+        const int jn = (j + D) % (D + 1);
+        es[jn] = fetch(k + j + D, w, buf[jn]);
+        apply(es[j], r, buf[j]);
+      }
     }
   }
   DM3 U, W, Ssum;
@@ -415,12 +425,13 @@ template <typename T> static bool gf_step_tables(GFArgs<T> &a) {
         }
       }
     }
-    // padding step: a forward step from x + mu (one move from x) along
-    // direction 0, loaded by the last prefetch and never applied
-    a.tab[mu][k >> 2] |= (GF_FIRST | GF_PODD) << (8 * (k & 3));
+    // padding steps: forward steps from x + mu (one move from x) along
+    // direction 0, loaded by the last prefetches and never applied
+    for (int j = 0; j < GF_PAD; ++j)
+      a.tab[mu][(k + j) >> 2] |= (GF_FIRST | GF_PODD) << (8 * ((k + j) & 3));
     n = k;
   }
-  QPHIX_REQUIRE(n % 2 == 0 && n <= GF_MAX_STEPS, "gauge force: bad step table", false);
+  QPHIX_REQUIRE(n % 6 == 0 && n <= GF_MAX_STEPS, "gauge force: bad step table", false);
   a.nsteps = n;
   return true;
 }
@@ -456,9 +467,9 @@ static void gauge_force(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_ga
   a.order = order;
   const dim3 grid((unsigned)((gg.vh + GF_BLOCK - 1) / GF_BLOCK * 8));
   if (rows2)
-    hipLaunchKernelGGL((k_gauge_force<T, 6>), grid, dim3(GF_BLOCK), 0, ctx().stream, a);
+    hipLaunchKernelGGL((k_gauge_force<T, 6, 1>), grid, dim3(GF_BLOCK), 0, ctx().stream, a);
   else
-    hipLaunchKernelGGL((k_gauge_force<T, 9>), grid, dim3(GF_BLOCK), 0, ctx().stream, a);
+    hipLaunchKernelGGL((k_gauge_force<T, 9, 1>), grid, dim3(GF_BLOCK), 0, ctx().stream, a);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
   info->status = QPHIX_SUCCESS;
