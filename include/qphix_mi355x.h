@@ -53,6 +53,12 @@ int QPHIX_F3_gauge_links_recon(QPHIX_F3_GaugeField *G);
  * on == 0: plain launches (the same kernels in the same order, hence
  * identical results). */
 void QPHIX_MI355X_set_graphs(int on);
+/* on != 0 (default; QPHIX_MRHS_LDS=0 in the environment sets the initial
+ * value to 0): multi-RHS Dslash launches with one source per work-item stage
+ * each wavefront's links through LDS (loaded once per site instead of once
+ * per source); on == 0: every work-item loads its own links.  Both give
+ * identical results. */
+void QPHIX_MI355X_set_mrhs_lds(int on);
 /* Storage chosen for L: 12 or 18. */
 int QPHIX_D3_asqtad_links_recon(QPHIX_D3_FermionLinksAsqtad *L);
 int QPHIX_F3_asqtad_links_recon(QPHIX_F3_FermionLinksAsqtad *L);

@@ -32,11 +32,16 @@ __device__ __forceinline__ void recon_row2(cplx<C> *U, C s) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-    // (r0 x r1)_c = r0[c1] r1[c2] - r0[c2] r1[c1]
-    C re = U[c1].re * U[3 + c2].re - U[c1].im * U[3 + c2].im - U[c2].re * U[3 + c1].re +
-           U[c2].im * U[3 + c1].im;
-    C im = U[c1].re * U[3 + c2].im + U[c1].im * U[3 + c2].re - U[c2].re * U[3 + c1].im -
-           U[c2].im * U[3 + c1].re;
+    // (r0 x r1)_c = r0[c1] r1[c2] - r0[c2] r1[c1], as explicit fmas: the
+    // rounding is then fixed by this code, not by where the compiler fuses
+    C re = U[c1].re * U[3 + c2].re;
+    re = fma(-U[c1].im, U[3 + c2].im, re);
+    re = fma(-U[c2].re, U[3 + c1].re, re);
+    re = fma(U[c2].im, U[3 + c1].im, re);
+    C im = U[c1].re * U[3 + c2].im;
+    im = fma(U[c1].im, U[3 + c2].re, im);
+    im = fma(-U[c2].re, U[3 + c1].im, im);
+    im = fma(-U[c2].im, U[3 + c1].re, im);
     U[6 + c].re = s * re;
     U[6 + c].im = -s * im;
   }

@@ -63,14 +63,12 @@ struct XcdWalk {
   }
 };
 
-// One direction's link pair applied to K sources (r0 .. r0+K-1) of the
-// neighbours nf (forward) and nb (back).
+// One direction's link pair u (forward link, then the back link; NC = 18, or
+// 12 = two rows each) applied to K sources (r0 .. r0+K-1) of the neighbours nf
+// (forward) and nb (back).
 template <typename T, int NC, int NR, int K>
-__device__ __forceinline__ void apply_pair_k(const typename vec16<T>::type *base, int ii, const T *in,
-                                             int nf, int nb, int r0, unsigned sg_f, unsigned sg_b,
-                                             cplx<T> (*acc)[3]) {
-  cplx<T> u[NC];
-  PairLoad<T, NC, true>::load(base, LINK_TILE, ii, u);
+__device__ __forceinline__ void apply_links_k(cplx<T> *u, const T *in, int nf, int nb, int r0, unsigned sg_f,
+                                              unsigned sg_b, cplx<T> (*acc)[3]) {
   cplx<T> vf[K][3], vb[K][3];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -96,6 +94,16 @@ __device__ __forceinline__ void apply_pair_k(const typename vec16<T>::type *base
 #pragma unroll
     for (int k = 0; k < K; ++k) matvec_acc(w, vb[k], acc[k]);
   }
+}
+
+// The same with the link pair loaded from the tile-major link array.
+template <typename T, int NC, int NR, int K>
+__device__ __forceinline__ void apply_pair_k(const typename vec16<T>::type *base, int ii, const T *in,
+                                             int nf, int nb, int r0, unsigned sg_f, unsigned sg_b,
+                                             cplx<T> (*acc)[3]) {
+  cplx<T> u[NC];
+  PairLoad<T, NC, true>::load(base, LINK_TILE, ii, u);
+  apply_links_k<T, NC, NR, K>(u, in, nf, nb, r0, sg_f, sg_b, acc);
 }
 
 // Per-source sums over the block: lane l holds K values of source group
@@ -173,9 +181,10 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs(const MrhsArgsT<T>
         Spin<T>::load(a.aux, slot, p);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          acc[k][c].re = acc[k][c].re + a.s * p[c].re;
-          acc[k][c].im = acc[k][c].im + a.s * p[c].im;
-          v[k] += (double)p[c].re * acc[k][c].re + (double)p[c].im * acc[k][c].im;
+          acc[k][c].re = fma(a.s, p[c].re, acc[k][c].re);
+          acc[k][c].im = fma(a.s, p[c].im, acc[k][c].im);
+          v[k] = fma((double)p[c].re, (double)acc[k][c].re, v[k]);
+          v[k] = fma((double)p[c].im, (double)acc[k][c].im, v[k]);
         }
       }
       Spin<T>::store(a.out, slot, acc[k]);
@@ -191,6 +200,149 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs(const MrhsArgsT<T>
 template <int NR, int BLOCK> __device__ __forceinline__ double lane_block_sum(double v) {
   double w[1] = {v};
   return group_block_sum<NR, 1, BLOCK>(w);
+}
+
+// One 16-byte link chunk, streamed (nontemporal) load.
+__device__ __forceinline__ float4 nt_chunk(const float4 *p) {
+  const nf4 d = __builtin_nontemporal_load(reinterpret_cast<const nf4 *>(p));
+  return make_float4(d.x, d.y, d.z, d.w);
+}
+__device__ __forceinline__ double2 nt_chunk(const double2 *p) {
+  const nd2 d = __builtin_nontemporal_load(reinterpret_cast<const nd2 *>(p));
+  return make_double2(d.x, d.y);
+}
+
+// Orders a wavefront's LDS accesses across its lanes (its LDS operations
+// execute in order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One source per work-item (K = 1) with the links staged through LDS.  With
+// NR lanes per site, loading links per lane fetches every link NR times from
+// L1 into registers, and that per-lane data return, not HBM, bounds the
+// kernel.  Here each wavefront covers SW = 64 / NR consecutive (logical) sites;
+// per direction it loads their fat and Naik link pairs once, as 16-byte chunks
+// spread over its 64 lanes (coalesced), stores them in its own LDS slice, and
+// every lane reads its site's pair from there (SW distinct addresses per read:
+// broadcasts).  The next direction's chunks are loaded into registers while
+// the current one is applied.  The slice is private to the wavefront, so no
+// block barrier is needed.  Arithmetic and its order are those of
+// k_dslash_mrhs<T, NL, NR, 1, EPI>.
+template <typename T, int NL, int NR, int EPI>
+__global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds(const MrhsArgsT<T> a) {
+  static_assert(NR >= 2 && 64 % NR == 0, "NR lanes per site, whole sites per wavefront");
+  using V = typename vec16<T>::type;
+  constexpr int CPV = vec16<T>::ncplx;
+  constexpr int NCH_F = 18 / CPV, NCH_L = NL / CPV, NCH = NCH_F + NCH_L;  // chunks per site and direction
+  constexpr int SW = 64 / NR;                                             // sites per wavefront
+  constexpr int CW = SW * NCH;                                            // chunks per wavefront and direction
+  constexpr int QL = (CW + 63) / 64;                                      // chunks loaded per lane
+  __shared__ V lds[DSLASH_BLOCK / 64][CW];
+  const unsigned mask = rhs_mask<NR>(a.states);
+  if (mask == 0) return;  // uniform over the grid
+  const int vh = a.g.vh;
+  const int lane = threadIdx.x & 63;
+  V *wl = lds[threadIdx.x >> 6];
+  const XcdWalk wk(a.nlog);
+  double v = 0.0;
+  for (int step = 0;; ++step) {
+    const int blk = wk.at(step);
+    if (blk < 0) break;
+    const int L0 = (blk * DSLASH_BLOCK + (threadIdx.x & ~63)) / NR;  // first site of the wavefront
+    if (L0 >= vh) continue;                                           // uniform per wavefront
+    // chunks this lane loads: c = lane + 64 q -> chunk k of wavefront site s
+    // (LDS slot k * SW + s); site -1 = none
+    int lsite[QL], lk[QL];
+#pragma unroll
+    for (int q = 0; q < QL; ++q) {
+      const int c = lane + 64 * q, k = c / SW, Ls = L0 + c % SW;
+      lk[q] = k;
+      lsite[q] = (c < CW && Ls < vh) ? ordered_site(a.g, a.order, Ls) : -1;
+    }
+    auto fetch = [&](int mu, V *pre) {
+#pragma unroll
+      for (int q = 0; q < QL; ++q) {
+        const int i = lsite[q], k = lk[q];
+        if (i < 0) continue;
+        pre[q] = k < NCH_F ? nt_chunk(a.fat + link_chunk(mu, k, i, NCH_F))
+                           : nt_chunk(a.lng + link_chunk(mu, k - NCH_F, i, NCH_L));
+      }
+    };
+    // this lane's (site, source)
+    const int s = lane / NR, r0 = lane % NR, L = L0 + s;
+    const bool active = L < vh && ((mask >> r0) & 1u);
+    int i = 0;
+    SiteNbr n{};
+    unsigned sg = 0;
+    if (active) {
+      i = ordered_site(a.g, a.order, L);
+      site_neighbours(a.g, i, a.cb, n);
+      if constexpr (NL == 12) sg = a.lsign[i];
+    }
+    cplx<T> acc[1][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[0][c] = cplx<T>{0, 0};
+    V pre[QL];
+    fetch(0, pre);
+    // unrolled: the neighbour indices are selected with constant indices
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      wave_lds_sync();  // the previous direction's reads are done
+#pragma unroll
+      for (int q = 0; q < QL; ++q)
+        if (lsite[q] >= 0) wl[lk[q] * SW + (lane + 64 * q) % SW] = pre[q];
+      wave_lds_sync();
+      if (mu < 3) fetch(mu + 1, pre);
+      if (active) {
+        cplx<T> u[18];
+#pragma unroll
+        for (int k = 0; k < NCH_F; ++k) {
+          const V d = wl[k * SW + s];
+          if constexpr (CPV == 2) {
+            u[2 * k] = cplx<T>{d.x, d.y};
+            u[2 * k + 1] = cplx<T>{d.z, d.w};
+          } else {
+            u[k] = cplx<T>{d.x, d.y};
+          }
+        }
+        apply_links_k<T, 18, NR, 1>(u, a.in, n.f1[mu], n.b1[mu], r0, 0, 0, acc);
+#pragma unroll
+        for (int k = 0; k < NCH_L; ++k) {
+          const V d = wl[(NCH_F + k) * SW + s];
+          if constexpr (CPV == 2) {
+            u[2 * k] = cplx<T>{d.x, d.y};
+            u[2 * k + 1] = cplx<T>{d.z, d.w};
+          } else {
+            u[k] = cplx<T>{d.x, d.y};
+          }
+        }
+        apply_links_k<T, NL, NR, 1>(u, a.in, n.f3[mu], n.b3[mu], r0, (sg >> (2 * mu + 1)) & 1u,
+                                    (sg >> (2 * mu)) & 1u, acc);
+      }
+    }
+    if (active) {
+      const int slot = i * NR + r0;
+      if constexpr (EPI == EPI_PKP) {
+        cplx<T> p[3];
+        Spin<T>::load(a.aux, slot, p);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          acc[0][c].re = fma(a.s, p[c].re, acc[0][c].re);
+          acc[0][c].im = fma(a.s, p[c].im, acc[0][c].im);
+          v = fma((double)p[c].re, (double)acc[0][c].re, v);
+          v = fma((double)p[c].im, (double)acc[0][c].im, v);
+        }
+      }
+      Spin<T>::store(a.out, slot, acc[0]);
+    }
+  }
+  if constexpr (EPI == EPI_PKP) {
+    const double sum = lane_block_sum<NR, DSLASH_BLOCK>(v);
+    if (threadIdx.x < NR) a.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sum;
+  }
 }
 
 } // namespace qphix

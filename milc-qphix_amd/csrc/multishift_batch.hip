@@ -214,14 +214,34 @@ inline int mrhs_nlog(int vh, int NR, int K) {
   return (int)(((size_t)vh * (NR / K) + DSLASH_BLOCK - 1) / DSLASH_BLOCK);
 }
 
-template <typename T, int NR, int K, int NL, int EPI> static int mrhs_grid_of(int vh) {
-  static int g = 0;  // per instantiation; the lattice is fixed between QPHIX_init and QPHIX_finalize
-  static int g_vh = -1;
-  if (g_vh != vh) {
-    g = resident_grid(k_dslash_mrhs<T, NL, NR, K, EPI>, mrhs_nlog(vh, NR, K));
-    g_vh = vh;
+// One source per work-item with NR >= 2 lanes per site uses the LDS-staged
+// link kernel (k_dslash_mrhs_lds; bit-identical results) unless switched off
+// (QPHIX_MI355X_set_mrhs_lds(0), or QPHIX_MRHS_LDS=0 before the first call).
+static int g_mrhs_lds = -1;
+static bool mrhs_lds_on() {
+  if (g_mrhs_lds < 0) {
+    const char *e = std::getenv("QPHIX_MRHS_LDS");
+    g_mrhs_lds = (e && std::atoi(e) == 0) ? 0 : 1;
   }
-  return g;
+  return g_mrhs_lds == 1;
+}
+
+// Resident grid of the kernel a launch uses (cached per kernel; the lattice is
+// fixed between QPHIX_init and QPHIX_finalize).
+template <typename T, int NR, int K, int NL, int EPI> static int mrhs_grid_of(int vh) {
+  const int v = (K == 1 && NR >= 2 && mrhs_lds_on()) ? 1 : 0;
+  static int g[2] = {0, 0};
+  static int g_vh[2] = {-1, -1};
+  if (g_vh[v] != vh) {
+    if constexpr (K == 1 && NR >= 2) {
+      if (v) g[v] = resident_grid(k_dslash_mrhs_lds<T, NL, NR, EPI>, mrhs_nlog(vh, NR, K));
+      else g[v] = resident_grid(k_dslash_mrhs<T, NL, NR, K, EPI>, mrhs_nlog(vh, NR, K));
+    } else {
+      g[v] = resident_grid(k_dslash_mrhs<T, NL, NR, K, EPI>, mrhs_nlog(vh, NR, K));
+    }
+    g_vh[v] = vh;
+  }
+  return g[v];
 }
 
 template <typename T, int NR, int K> static int mrhs_blocks_k(int vh, int recon, int epi) {
@@ -241,6 +261,27 @@ template <typename T, int NR, int K>
 static void mrhs_launch(const LinksT<T> &L, int epi, MrhsArgsT<T> a) {
   const int vh = a.g.vh;
   a.nlog = mrhs_nlog(vh, NR, K);
+  if constexpr (K == 1 && NR >= 2) {
+    if (mrhs_lds_on()) {
+      const dim3 blk(DSLASH_BLOCK);
+      if (L.recon == 12) {
+        if (epi == EPI_PLAIN)
+          hipLaunchKernelGGL((k_dslash_mrhs_lds<T, 12, NR, EPI_PLAIN>), dim3(mrhs_grid_of<T, NR, K, 12, EPI_PLAIN>(vh)),
+                             blk, 0, ctx().stream, a);
+        else
+          hipLaunchKernelGGL((k_dslash_mrhs_lds<T, 12, NR, EPI_PKP>), dim3(mrhs_grid_of<T, NR, K, 12, EPI_PKP>(vh)),
+                             blk, 0, ctx().stream, a);
+      } else {
+        if (epi == EPI_PLAIN)
+          hipLaunchKernelGGL((k_dslash_mrhs_lds<T, 18, NR, EPI_PLAIN>), dim3(mrhs_grid_of<T, NR, K, 18, EPI_PLAIN>(vh)),
+                             blk, 0, ctx().stream, a);
+        else
+          hipLaunchKernelGGL((k_dslash_mrhs_lds<T, 18, NR, EPI_PKP>), dim3(mrhs_grid_of<T, NR, K, 18, EPI_PKP>(vh)),
+                             blk, 0, ctx().stream, a);
+      }
+      return;
+    }
+  }
   if (L.recon == 12) {
     if (epi == EPI_PLAIN)
       hipLaunchKernelGGL((k_dslash_mrhs<T, 12, NR, K, EPI_PLAIN>), dim3(mrhs_grid_of<T, NR, K, 12, EPI_PLAIN>(vh)),
@@ -396,6 +437,8 @@ static double dslash_multi_api(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVec
 using namespace qphix;
 
 extern "C" {
+
+void QPHIX_MI355X_set_mrhs_lds(int on) { g_mrhs_lds = on ? 1 : 0; }
 
 void QPHIX_F3_asqtad_dslash_multi(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVector *out[],
                                   QPHIX_F3_ColorVector *in[], int nvec, QPHIX_evenodd_t parity) {

@@ -127,6 +127,7 @@ class Lib:
         _sig(lib, "QPHIX_MI355X_device_info", C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_double)])
         _sig(lib, "QPHIX_MI355X_set_long_recon", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_set_gauge_recon", None, [C.c_int])
+        _sig(lib, "QPHIX_MI355X_set_mrhs_lds", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_set_graphs", None, [C.c_int])
         _sig(lib, "QPHIX_D3_asqtad_links_backcheck", None, [_dp, C.POINTER(C.c_double)])
         _sig(lib, "QPHIX_MI355X_copy_time", C.c_double, [C.c_size_t, C.c_int])

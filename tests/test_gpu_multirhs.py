@@ -117,6 +117,56 @@ def test_multi_rhs_graph_replay_matches_plain_launches():
         o.free()
 
 
+def test_lds_staged_links_match_per_lane_loads():
+    """The multi-RHS Dslash with links staged through LDS and the one that
+    loads them per work-item give bit-identical results: plain Dslash on 8 and
+    4 FP32 vectors, and a batched multi-shift solve (FP64 and FP32 phases, the
+    p.Ap epilogue, refinement)."""
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    raw = [F.to_raw(a) for a in random_links(lat, "weak", 1011)]
+    L32 = A.Links(s, *[r.astype(np.float32) for r in raw], prec="F")
+    L = A.Links(s, *raw)
+    srcs = [F.to_raw(F.gaussian_source(lat, 500 + k), np.float32) for k in range(8)]
+    vin = [A.Vector(s, x, A.EVENODD, "F") for x in srcs]
+    masses = [0.03, 0.2]
+    bs = [F.to_raw(F.gaussian_source(lat, 520 + r, parity=0)) for r in range(8)]
+    vb = [A.Vector(s, b, A.EVEN) for b in bs]
+    got = {}
+    try:
+        for on in (1, 0):
+            s.lib.QPHIX_MI355X_set_mrhs_lds(on)
+            print(f"[case] mrhs lds={on}", flush=True)
+            outs = []
+            for n in (8, 4):
+                vout = [A.Vector(s, None, A.EVENODD, "F") for _ in range(n)]
+                A.dslash_multi(L32, vout, vin[:n], A.EVEN)
+                outs.append([v.extract() for v in vout])
+                for v in vout:
+                    v.free()
+            vx = [[A.Vector(s, None, A.EVEN) for _ in masses] for _ in bs]
+            ret, info, res = A.invert_multi_rhs(L, vx, vb, masses, A.EVEN, 1e-18, niter=1000,
+                                                nrestart=2, rhs_block=8)
+            assert info.status == 0
+            got[on] = (outs, ret, [[x.extract() for x in row] for row in vx],
+                       [[(r.final_iter, r.final_rsq) for r in row] for row in res])
+            for row in vx:
+                for x in row:
+                    x.free()
+    finally:
+        s.lib.QPHIX_MI355X_set_mrhs_lds(1)
+    for a, b in zip(got[1][0], got[0][0]):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    assert got[1][1] == got[0][1] and got[1][3] == got[0][3]
+    for r1, r0 in zip(got[1][2], got[0][2]):
+        for x, y in zip(r1, r0):
+            np.testing.assert_array_equal(x, y)
+    for o in [L, L32] + vin + vb:
+        o.free()
+
+
 def test_multi_rhs_ragged_and_zero_source():
     """5 sources in a block of 8 (three empty slots), one of them zero: the
     zero source returns zero solutions, the others meet their targets."""
