@@ -81,17 +81,21 @@ def cpu_baseline(dims, raw, src, budget_s):
 
 
 def cg_compare(A, s, L64, lat, src):
-    """FP64 CG vs mixed-precision CG (FP32 and FP16-stored sloppy spinors, FP64
-    reliable updates), each to a 1e-10 true relative residual (resid is the
-    squared residual, 1e-20), m = 0.02 in M = m^2 - D_eo D_oe (API mass m/2)."""
+    """FP64 CG vs mixed-precision CG (FP32 and FP16-stored sloppy spinors, FP32
+    or 16-bit fixed-point sloppy links, FP64 reliable updates), each to a 1e-10
+    true relative residual (resid is the squared residual, 1e-20), m = 0.02 in
+    M = m^2 - D_eo D_oe (API mass m/2)."""
     mass, resid = 0.01, 1e-20
     b = src.copy()
     b[6 * lat.vh:] = 0
     vb = A.Vector(s, b, A.EVEN, "D")
     out = {}
     x64 = None
-    for name, bits in (("f64", None), ("mixed_f32", 32), ("mixed_f16", 16)):
+    for name, bits, lbits in (("f64", None, None), ("mixed_f32", 32, 32), ("mixed_f16", 16, 32),
+                              ("mixed_f32_l16", 32, 16), ("mixed_f16_l16", 16, 16)):
         vx = A.Vector(s, None, A.EVEN, "D")
+        if lbits is not None:
+            s.lib.QPHIX_MI355X_set_sloppy_link_bits(lbits)
         t = time.perf_counter()
         if bits is None:
             ret, info, res = A.invert(L64, vx, vb, mass, A.EVEN, resid, niter=5000, nrestart=10)
@@ -99,6 +103,7 @@ def cg_compare(A, s, L64, lat, src):
             ret, info, res = A.invert_mixed(L64, vx, vb, mass, A.EVEN, resid, sloppy_bits=bits,
                                             niter=5000, nrestart=10)
         sec = time.perf_counter() - t
+        s.lib.QPHIX_MI355X_set_sloppy_link_bits(32)
         x = vx.extract()[:6 * lat.vh]
         rec = {"seconds": round(sec, 4), "iters": res.final_iter, "final_rsq": res.final_rsq,
                "status": info.status}
@@ -395,6 +400,16 @@ def main():
                            "bytes_per_site": rb32, "recon": L32.recon}
     log(f"[bench] FP32 Dslash {ms32 * 1e3:.1f} us  {extra['dslash_f32']['GBs']} GB/s")
     del raw32
+
+    # Sloppy Dslash with 16-bit fixed-point links (per link 16-bit elements and
+    # a float scale: 80 B per fat pair, 64 B per 12-real Naik pair), FP32 vectors
+    A.dslash_sloppy(L64, vout32, vin32, A.EVEN, 16, args.warmup)
+    ms16 = A.dslash_sloppy(L64, vout32, vin32, A.EVEN, 16, args.steps)
+    rb16 = 4 * 80 + 4 * (64 if L64.recon == 12 else 80) + (1 if L64.recon == 12 else 0) + 2 * 24
+    extra["dslash_l16"] = {"us": round(ms16 * 1e3, 2),
+                           "GBs": round(rb16 * lat.vh / (ms16 * 1e-3) / 1e9, 1),
+                           "bytes_per_site": rb16, "recon": L64.recon}
+    log(f"[bench] 16-bit-link Dslash {ms16 * 1e3:.1f} us  {extra['dslash_l16']['GBs']} GB/s")
 
     if not args.no_cg:
         extra["cg"] = cg_compare(A, s, L64, lat, src)

@@ -59,6 +59,18 @@ void QPHIX_MI355X_set_graphs(int on);
  * per source); on == 0: every work-item loads its own links.  Both give
  * identical results. */
 void QPHIX_MI355X_set_mrhs_lds(int on);
+/* Links of the mixed solvers' sloppy operator (QPHIX_D3_asqtad_invert_mixed):
+ * 32 (default) = FP32, 16 = 16-bit fixed point (per link a float scale, the
+ * largest |element|, and int16 elements; ~1.5e-5 relative per element), FP32
+ * arithmetic either way.  The FP64 reliable updates keep the result at the
+ * requested true residual. */
+void QPHIX_MI355X_set_sloppy_link_bits(int bits);
+/* One application of the sloppy Dslash (the D3 links' FP32 or 16-bit copy,
+ * link_bits 32 or 16) on FP32 vectors: out = D in on `parity`, repeated reps
+ * times; returns milliseconds per application (-1 on a contract violation). */
+double QPHIX_D3_asqtad_dslash_sloppy(QPHIX_D3_FermionLinksAsqtad *L, QPHIX_F3_ColorVector *out,
+                                     QPHIX_F3_ColorVector *in, QPHIX_evenodd_t parity, int link_bits,
+                                     int reps);
 /* Storage chosen for L: 12 or 18. */
 int QPHIX_D3_asqtad_links_recon(QPHIX_D3_FermionLinksAsqtad *L);
 int QPHIX_F3_asqtad_links_recon(QPHIX_F3_FermionLinksAsqtad *L);

@@ -1,6 +1,7 @@
 // common.h -- internal declarations shared by the HIP sources of
 // libqphixmilc_hip.so.  Not installed; callers see include/qphix*.h only.
 #pragma once
+#include <type_traits>
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
@@ -109,6 +110,24 @@ template <typename T> struct vec16;
 template <> struct vec16<double> { using type = double2; static constexpr int ncplx = 1; };
 template <> struct vec16<float> { using type = float4; static constexpr int ncplx = 2; };
 
+// 16-bit fixed-point links (storage of the sloppy operators of the mixed
+// solvers, arithmetic in FP32).  Each link has a scale s = max |element| (over
+// the real and imaginary parts of its stored rows) and elements
+// q = round(x / s * 32767), x = q * s / 32767: |error| <= s / 65534 per
+// element.  A pair (forward link, then back link) of NC complex elements is
+// cut into 16-byte chunks: the 2 NC int16 values in order, 8 to a chunk, and
+// the two float scales in the last two dwords of the last chunk (NC = 18:
+// 5 chunks, 80 B; NC = 12: 4 chunks, 64 B).
+struct Q16 {};
+template <> struct vec16<Q16> { using type = uint4; static constexpr int ncplx = 0; };
+// chunks of a link pair with nc complex elements
+template <typename TL> __host__ __device__ constexpr int link_nch(int nc) {
+  return std::is_same<TL, Q16>::value ? (nc == 12 ? 4 : 5) : nc / vec16<TL>::ncplx;
+}
+// arithmetic type of a link storage type
+template <typename TL> struct link_compute { using type = TL; };
+template <> struct link_compute<Q16> { using type = float; };
+
 #ifndef QPHIX_LINK_TILE
 #define QPHIX_LINK_TILE 64
 #endif
@@ -130,6 +149,7 @@ template <typename T> struct LinksT {
   int recon = 18;                          // 18 or 12 (long links)
   int parity = 0;                          // QPHIX_EVEN/ODD/EVENODD supplied
   LinksT<float> *sloppy = nullptr;         // lazily built FP32 copy (D3 only)
+  LinksT<Q16> *sloppy16 = nullptr;         // lazily built 16-bit copy (D3 only)
 };
 
 template <typename T> struct VectorT {
@@ -197,8 +217,22 @@ void launch_dslash(const LinksT<T> &L, int cb, T *out, const T *in, int epi, con
 void launch_dslash_half(const LinksT<float> &L, int cb, HalfSite *out_h, float *out_f,
                         const HalfSite *in, int epi, const DslashExtra &ex);
 
+// 16-bit fixed-point links (Q16) with FP32 or FP16-stored spinors and FP32
+// arithmetic: the sloppy operator of the mixed solvers.
+void launch_dslash_q16(const LinksT<Q16> &L, int cb, float *out, const float *in, int epi,
+                       const DslashExtra &ex);
+void launch_dslash_q16_half(const LinksT<Q16> &L, int cb, HalfSite *out_h, float *out_f,
+                            const HalfSite *in, int epi, const DslashExtra &ex);
+
 // FP32 copy of FP64 links in the same layout (fields.hip)
 void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf);
+// 16-bit fixed-point copy of FP64 links (fields.hip)
+void links_to_q16(const LinksT<double> &Ld, LinksT<Q16> &Lq);
+// The lazily built sloppy copies of a D3 link set (fields.hip)
+const LinksT<float> &sloppy_links(LinksT<double> *L);
+const LinksT<Q16> &sloppy_links16(LinksT<double> *L);
+// Link bits of the mixed solvers' sloppy operator: 32 (default) or 16.
+int sloppy_link_bits();
 
 // Mixed-precision CG on one parity system (mixed.hip); out is the initial guess.
 int mixed_solve(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,

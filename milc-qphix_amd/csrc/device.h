@@ -181,6 +181,28 @@ template <int NC, bool NT> struct PairLoad<float, NC, NT> {
   }
 };
 
+// 16-bit fixed-point pair (see Q16): NC complex elements decoded to FP32.
+typedef unsigned nu4 __attribute__((ext_vector_type(4)));
+template <int NC, bool NT> struct PairLoad<Q16, NC, NT> {
+  static __device__ __forceinline__ void load(const uint4 *base, int vh, int i, cplx<float> *u) {
+    constexpr int NCH = link_nch<Q16>(NC);
+    const nu4 *b = reinterpret_cast<const nu4 *>(base);
+    nu4 c[NCH];
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) c[k] = NT ? __builtin_nontemporal_load(b + (size_t)k * vh + i) : b[(size_t)k * vh + i];
+    const float sf = __uint_as_float(c[NCH - 1][2]) * (1.0f / 32767.0f);
+    const float sb = __uint_as_float(c[NCH - 1][3]) * (1.0f / 32767.0f);
+#pragma unroll
+    for (int e = 0; e < 2 * NC; ++e) {
+      const unsigned w = c[e / 8][(e % 8) / 2];
+      const int q = (e % 2) ? ((int)w >> 16) : (int)(short)(w & 0xffffu);
+      const float x = (float)q * (e < NC ? sf : sb);
+      if (e % 2) u[e / 2].im = x;
+      else u[e / 2].re = x;
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Lattice arithmetic.  cb = parity of the site i; the returned indices are
 // raw indices of the opposite parity.

@@ -69,7 +69,7 @@ static DslashKArgs<TL, SI, SO> make_args(const LinksT<TL> &L, int cb, SO *out, c
   a.out = out;
   a.g = make_geo();
   a.cb = cb;
-  a.s = (TL)ex.s;
+  a.s = (typename link_compute<TL>::type)ex.s;
   a.aux = reinterpret_cast<const SI *>(ex.aux);
   a.b = reinterpret_cast<const SI *>(ex.b);
   a.r = reinterpret_cast<SO *>(ex.r);
@@ -101,6 +101,23 @@ void launch_dslash_half(const LinksT<float> &L, int cb, HalfSite *out_h, float *
   else
     dispatch<float, HalfSite, float>(make_args<float, HalfSite, float>(L, cb, out_f, in, ex), L.recon, epi,
                                      ex.consumer_finalize);
+}
+
+void launch_dslash_q16(const LinksT<Q16> &L, int cb, float *out, const float *in, int epi,
+                       const DslashExtra &ex) {
+  QPHIX_REQUIRE(L.fat[cb] && L.lng[cb] && in && out, "Dslash: links or vectors of this parity missing");
+  dispatch<Q16, float, float>(make_args<Q16, float, float>(L, cb, out, in, ex), L.recon, epi, ex.consumer_finalize);
+}
+
+void launch_dslash_q16_half(const LinksT<Q16> &L, int cb, HalfSite *out_h, float *out_f, const HalfSite *in,
+                            int epi, const DslashExtra &ex) {
+  QPHIX_REQUIRE(L.fat[cb] && L.lng[cb] && in && (out_h || out_f), "Dslash: links or vectors of this parity missing");
+  if (out_h)
+    dispatch<Q16, HalfSite, HalfSite>(make_args<Q16, HalfSite, HalfSite>(L, cb, out_h, in, ex), L.recon, epi,
+                                      ex.consumer_finalize);
+  else
+    dispatch<Q16, HalfSite, float>(make_args<Q16, HalfSite, float>(L, cb, out_f, in, ex), L.recon, epi,
+                                   ex.consumer_finalize);
 }
 
 } // namespace qphix

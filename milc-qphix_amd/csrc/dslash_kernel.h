@@ -47,7 +47,7 @@ template <typename TL, typename SI, typename SO> struct DslashKArgs {
   SO *out;
   Geo g;
   int cb;
-  TL s;         // epilogue scalar
+  typename link_compute<TL>::type s;  // epilogue scalar
   const SI *aux;
   const SI *b;
   SO *r, *p;
@@ -76,40 +76,49 @@ __device__ __forceinline__ int block_slot(int opt) {
 template <typename TL, int NC, typename SI, bool NT>
 __device__ __forceinline__ void apply_pair(const typename vec16<TL>::type *base, int vh, int i,
                                            const SI *in, int nf, int nb, unsigned sg_f,
-                                           unsigned sg_b, cplx<TL> *acc) {
-  cplx<TL> u[NC];
+                                           unsigned sg_b, cplx<typename link_compute<TL>::type> *acc) {
+  using TC = typename link_compute<TL>::type;
+  cplx<TC> u[NC];
   PairLoad<TL, NC, NT>::load(base, vh, i, u);
-  cplx<TL> vf[3], vb[3];
+  cplx<TC> vf[3], vb[3];
   Spin<SI>::load(in, nf, vf);
   Spin<SI>::load(in, nb, vb);
   if constexpr (NC == 18) {
     matvec_acc(u, vf, acc);
     matvec_acc(u + 9, vb, acc);
   } else {
-    cplx<TL> w[9];
+    cplx<TC> w[9];
 #pragma unroll
     for (int k = 0; k < 6; ++k) w[k] = u[k];
-    recon_row2(w, sg_f ? (TL)-1 : (TL)1);
+    recon_row2(w, sg_f ? (TC)-1 : (TC)1);
     matvec_acc(w, vf, acc);
 #pragma unroll
     for (int k = 0; k < 6; ++k) w[k] = u[6 + k];
-    recon_row2(w, sg_b ? (TL)-1 : (TL)1);
+    recon_row2(w, sg_b ? (TC)-1 : (TC)1);
     matvec_acc(w, vb, acc);
   }
 }
 
+// Occupancy target (waves per SIMD) for k_dslash, for measurements; unset, the
+// compiler's register allocation sets it.
+#ifdef QPHIX_DSLASH_WAVES_PER_EU
+#define QPHIX_DSLASH_OCC __attribute__((amdgpu_waves_per_eu(QPHIX_DSLASH_WAVES_PER_EU)))
+#else
+#define QPHIX_DSLASH_OCC
+#endif
+
 template <typename TL, int NL, typename SI, typename SO, int EPI, int OPT>
-__global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash(const DslashKArgs<TL, SI, SO> a) {
+__global__ void __launch_bounds__(DSLASH_BLOCK) QPHIX_DSLASH_OCC k_dslash(const DslashKArgs<TL, SI, SO> a) {
   if (a.gate && *a.gate == 0) return;  // uniform over the grid
-  constexpr int CPV = vec16<TL>::ncplx;
-  constexpr int NCH_F = 18 / CPV, NCH_L = NL / CPV;
+  using TC = typename link_compute<TL>::type;
+  constexpr int NCH_F = link_nch<TL>(18), NCH_L = link_nch<TL>(NL);
   constexpr bool NTL = (OPT & OPT_NTL) != 0;
   const int vh = a.g.vh;
   const int i = block_slot(OPT) * blockDim.x + threadIdx.x;
   const bool valid = i < vh;
-  cplx<TL> acc[3];
+  cplx<TC> acc[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) acc[c] = cplx<TL>{0, 0};
+  for (int c = 0; c < 3; ++c) acc[c] = cplx<TC>{0, 0};
   if (valid) {
     SiteNbr n;
     site_neighbours(a.g, i, a.cb, n);
@@ -138,7 +147,7 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash(const DslashKArgs<TL, S
   } else if constexpr (EPI == EPI_PKP) {
     double v[1] = {0.0};
     if (valid) {
-      cplx<TL> p[3];
+      cplx<TC> p[3];
       Spin<SI>::load(a.aux, i, p);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -152,14 +161,14 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash(const DslashKArgs<TL, S
   } else {  // EPI_RESTART
     double v[2] = {0.0, 0.0};
     if (valid) {
-      cplx<TL> x[3], b[3];
+      cplx<TC> x[3], b[3];
       Spin<SI>::load(a.aux, i, x);
       Spin<SI>::load(a.b, i, b);
       double num = 0, den = 0;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const TL tr = acc[c].re + a.s * x[c].re;
-        const TL ti = acc[c].im + a.s * x[c].im;
+        const TC tr = acc[c].re + a.s * x[c].re;
+        const TC ti = acc[c].im + a.s * x[c].im;
         acc[c].re = b[c].re + tr;
         acc[c].im = b[c].im + ti;
         num += (double)acc[c].re * acc[c].re + (double)acc[c].im * acc[c].im;

@@ -176,6 +176,42 @@ __global__ void k_links_to_float(float *dst, const double *src, int vh, int nc_p
     }
 }
 
+// FP64 links (double2 chunks) -> 16-bit fixed point (Q16 chunks), per site
+// and direction one pair of nc_pair complex elements (forward link, then back
+// link), each link with its own scale.
+__global__ void k_links_to_q16(uint4 *dst, const double *src, int vh, int nc_pair) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  const int nch = nc_pair == 12 ? link_nch<Q16>(12) : link_nch<Q16>(18);
+  for (int mu = 0; mu < 4; ++mu) {
+    double x[36];
+    for (int k = 0; k < nc_pair; ++k) {
+      const double *c = src + link_chunk(mu, k, i, nc_pair) * 2;
+      x[2 * k] = c[0];
+      x[2 * k + 1] = c[1];
+    }
+    float sc[2];
+    for (int h = 0; h < 2; ++h) {
+      double m = 0;
+      for (int e = h * nc_pair; e < (h + 1) * nc_pair; ++e) m = fmax(m, fabs(x[e]));
+      float f = (float)m;
+      if ((double)f < m) f = nextafterf(f, INFINITY);  // |x| / f <= 1
+      sc[h] = f;
+    }
+    unsigned w[20] = {};
+    for (int e = 0; e < 2 * nc_pair; ++e) {
+      const float f = sc[e < nc_pair ? 0 : 1];
+      int q = f > 0.f ? (int)rint(x[e] / (double)f * 32767.0) : 0;
+      q = q > 32767 ? 32767 : (q < -32767 ? -32767 : q);
+      w[e / 2] |= ((unsigned)q & 0xffffu) << (16 * (e % 2));
+    }
+    w[4 * nch - 2] = __float_as_uint(sc[0]);
+    w[4 * nch - 1] = __float_as_uint(sc[1]);
+    for (int k = 0; k < nch; ++k)
+      dst[link_chunk(mu, k, i, nch)] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+  }
+}
+
 // Momentum raw (anti-Hermitian A, 18 reals) -> 8 reals of H = -iA
 // (qphix_d3.cpp:420-433): [ReH01, ImH01, ReH02, ImH02, ReH12, ImH12, H00, H11].
 template <typename T> __global__ void k_mom_in(T *h, const T *raw, int nlinks) {
@@ -237,9 +273,15 @@ template <typename T> void links_free(LinksT<T> *L) {
     delete L->sloppy;
     L->sloppy = nullptr;
   }
+  if (L->sloppy16) {
+    links_free(L->sloppy16);
+    delete L->sloppy16;
+    L->sloppy16 = nullptr;
+  }
 }
 template void links_free<double>(LinksT<double> *);
 template void links_free<float>(LinksT<float> *);
+template void links_free<Q16>(LinksT<Q16> *);
 
 void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf) {
   const int vh = ctx().g.vh;
@@ -261,6 +303,47 @@ void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf) {
   }
   HIP_CHECK(hipGetLastError());
 }
+
+void links_to_q16(const LinksT<double> &Ld, LinksT<Q16> &Lq) {
+  const int vh = ctx().g.vh;
+  Lq.recon = Ld.recon;
+  Lq.parity = Ld.parity;
+  const int ncl = Ld.recon == 12 ? 12 : 18;
+  for (int cb = 0; cb < 2; ++cb) {
+    if (!Ld.fat[cb]) continue;
+    Lq.fat[cb] = (Q16 *)dev_alloc(16 * 4 * (size_t)link_nch<Q16>(18) * link_sites(vh));
+    Lq.lng[cb] = (Q16 *)dev_alloc(16 * 4 * (size_t)link_nch<Q16>(ncl) * link_sites(vh));
+    hipLaunchKernelGGL(k_links_to_q16, dim3(nblk(vh)), dim3(256), 0, ctx().stream,
+                       reinterpret_cast<uint4 *>(Lq.fat[cb]), Ld.fat[cb], vh, 18);
+    hipLaunchKernelGGL(k_links_to_q16, dim3(nblk(vh)), dim3(256), 0, ctx().stream,
+                       reinterpret_cast<uint4 *>(Lq.lng[cb]), Ld.lng[cb], vh, ncl);
+    if (Ld.lsign[cb]) {
+      Lq.lsign[cb] = (uint8_t *)dev_alloc((size_t)vh);
+      HIP_CHECK(hipMemcpyAsync(Lq.lsign[cb], Ld.lsign[cb], (size_t)vh, hipMemcpyDeviceToDevice,
+                               ctx().stream));
+    }
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+const LinksT<float> &sloppy_links(LinksT<double> *L) {
+  if (!L->sloppy) {
+    L->sloppy = new LinksT<float>;
+    links_to_float(*L, *L->sloppy);
+  }
+  return *L->sloppy;
+}
+
+const LinksT<Q16> &sloppy_links16(LinksT<double> *L) {
+  if (!L->sloppy16) {
+    L->sloppy16 = new LinksT<Q16>;
+    links_to_q16(*L, *L->sloppy16);
+  }
+  return *L->sloppy16;
+}
+
+static int g_sloppy_link_bits = 32;
+int sloppy_link_bits() { return g_sloppy_link_bits; }
 
 // ------------------------------------------------------------------ V
 template <typename T>
@@ -648,6 +731,7 @@ QPHIX_FIELD_API(F, float)
 
 void QPHIX_MI355X_set_long_recon(int mode) { g_recon_mode = mode == 18 ? 18 : 0; }
 void QPHIX_MI355X_set_gauge_recon(int mode) { g_gauge_recon_mode = mode == 18 ? 18 : 0; }
+void QPHIX_MI355X_set_sloppy_link_bits(int bits) { g_sloppy_link_bits = bits == 16 ? 16 : 32; }
 
 void QPHIX_D3_asqtad_links_backcheck(QPHIX_D3_FermionLinksAsqtad *L, double out[4]) {
   backcheck<double>(L, out);

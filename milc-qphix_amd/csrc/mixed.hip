@@ -262,18 +262,50 @@ static void sloppy_apply(const LinksT<float> &Lf, int parity, HalfSite *p, HalfS
   launch_dslash_half(Lf, parity, nullptr, Ap, tmp, EPI_PKP, e2);
 }
 
-template <typename P>
+// The same with 16-bit fixed-point links.
+static void sloppy_apply(const LinksT<Q16> &Lq, int parity, float *p, float *tmp, float *Ap, double s,
+                         SolverState *st, const int *gate) {
+  DslashExtra e1;
+  e1.gate = gate;
+  launch_dslash_q16(Lq, 1 - parity, tmp, p, EPI_PLAIN, e1);
+  DslashExtra e2;
+  e2.gate = gate;
+  e2.s = s;
+  e2.aux = p;
+  e2.state = st;
+  e2.scalar_op = SOP_MX_PAP;
+  e2.consumer_finalize = true;
+  launch_dslash_q16(Lq, parity, Ap, tmp, EPI_PKP, e2);
+}
+
+static void sloppy_apply(const LinksT<Q16> &Lq, int parity, HalfSite *p, HalfSite *tmp, float *Ap, double s,
+                         SolverState *st, const int *gate) {
+  DslashExtra e1;
+  e1.gate = gate;
+  launch_dslash_q16_half(Lq, 1 - parity, tmp, nullptr, p, EPI_PLAIN, e1);
+  DslashExtra e2;
+  e2.gate = gate;
+  e2.s = s;
+  e2.aux = p;
+  e2.state = st;
+  e2.scalar_op = SOP_MX_PAP;
+  e2.consumer_finalize = true;
+  launch_dslash_q16_half(Lq, parity, nullptr, Ap, tmp, EPI_PKP, e2);
+}
+
+// P: storage of the sloppy vectors (float or HalfSite); LK: of the sloppy
+// links (float, or Q16 16-bit fixed point).
+template <typename P, typename LK>
 static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
                         QPHIX_resid_arg_t *res, double mass, VectorT<double> *out,
                         VectorT<double> *in) {
   const double t0 = wall_now();
   const int vh = ctx().g.vh;
   const size_t nv = 6 * (size_t)vh;
-  if (!L->sloppy) {
-    L->sloppy = new LinksT<float>;
-    links_to_float(*L, *L->sloppy);
-  }
-  const LinksT<float> &Lf = *L->sloppy;
+  const LinksT<LK> &Lf = [&]() -> const LinksT<LK> & {
+    if constexpr (std::is_same<LK, Q16>::value) return sloppy_links16(L);
+    else return sloppy_links(L);
+  }();
   const size_t psz = sizeof(P) == sizeof(HalfSite) ? sizeof(HalfSite) * (size_t)vh : nv * sizeof(float);
   char *ws = (char *)ws_get(2 * nv * sizeof(double) + 2 * psz + 3 * nv * sizeof(float) +
                             sizeof(SolverState) + 1024);
@@ -379,11 +411,23 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
   return iteration;
 }
 
+// Sloppy vectors of sloppy_bits (32: FP32, 16: FP16-stored); sloppy links of
+// sloppy_link_bits() (32: FP32, 16: 16-bit fixed point).
+static int mixed_dispatch(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
+                          QPHIX_resid_arg_t *res, double mass, VectorT<double> *out, VectorT<double> *in,
+                          int sloppy_bits) {
+  const bool l16 = sloppy_link_bits() == 16;
+  if (sloppy_bits == 16)
+    return l16 ? invert_mixed<HalfSite, Q16>(info, L, inv, res, mass, out, in)
+               : invert_mixed<HalfSite, float>(info, L, inv, res, mass, out, in);
+  return l16 ? invert_mixed<float, Q16>(info, L, inv, res, mass, out, in)
+             : invert_mixed<float, float>(info, L, inv, res, mass, out, in);
+}
+
 int mixed_solve(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
                 QPHIX_resid_arg_t *res, double mass, VectorT<double> *out, VectorT<double> *in,
                 int sloppy_bits) {
-  if (sloppy_bits == 16) return invert_mixed<HalfSite>(info, L, inv, res, mass, out, in);
-  return invert_mixed<float>(info, L, inv, res, mass, out, in);
+  return mixed_dispatch(info, L, inv, res, mass, out, in, sloppy_bits);
 }
 
 } // namespace qphix
@@ -391,6 +435,41 @@ int mixed_solve(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
 using namespace qphix;
 
 extern "C" {
+
+double QPHIX_D3_asqtad_dslash_sloppy(QPHIX_D3_FermionLinksAsqtad *L, QPHIX_F3_ColorVector *out,
+                                     QPHIX_F3_ColorVector *in, QPHIX_evenodd_t parity, int link_bits,
+                                     int reps) {
+  QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called: asqtad_dslash_sloppy", -1.0);
+  QPHIX_REQUIRE(parity == QPHIX_EVEN || parity == QPHIX_ODD, "asqtad_dslash_sloppy: parity must be EVEN or ODD",
+                -1.0);
+  QPHIX_REQUIRE(link_bits == 32 || link_bits == 16, "asqtad_dslash_sloppy: link_bits must be 32 or 16", -1.0);
+  const int cb = parity == QPHIX_EVEN ? 0 : 1;
+  QPHIX_REQUIRE(L && out && in && out->half[cb] && in->half[1 - cb] && L->fat[cb],
+                "asqtad_dslash_sloppy: bad arguments", -1.0);
+  if (reps < 1) reps = 1;
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  DslashExtra ex;
+  if (link_bits == 16) {
+    const LinksT<Q16> &Lq = sloppy_links16(L);
+    launch_dslash_q16(Lq, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
+    HIP_CHECK(hipEventRecord(e0, ctx().stream));
+    for (int r = 0; r < reps; ++r) launch_dslash_q16(Lq, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
+  } else {
+    const LinksT<float> &Lf = sloppy_links(L);
+    launch_dslash<float>(Lf, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
+    HIP_CHECK(hipEventRecord(e0, ctx().stream));
+    for (int r = 0; r < reps; ++r) launch_dslash<float>(Lf, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
+  }
+  HIP_CHECK(hipEventRecord(e1, ctx().stream));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  return (double)ms / reps;
+}
 
 int QPHIX_D3_asqtad_invert_mixed(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad *L,
                                  QPHIX_invert_arg_t *inv, QPHIX_resid_arg_t *res, double mass,
@@ -407,8 +486,7 @@ int QPHIX_D3_asqtad_invert_mixed(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad
   for (int cb = 0; cb < 2; ++cb)
     if (inv->parity & (cb ? QPHIX_ODD : QPHIX_EVEN))
       QPHIX_REQUIRE(out->half[cb] && in->half[cb], "asqtad_invert_mixed: vector lacks a solved parity", 0);
-  if (sloppy_bits == 32) return invert_mixed<float>(info, L, inv, res, mass, out, in);
-  return invert_mixed<HalfSite>(info, L, inv, res, mass, out, in);
+  return mixed_dispatch(info, L, inv, res, mass, out, in, sloppy_bits);
 }
 
 } // extern "C"

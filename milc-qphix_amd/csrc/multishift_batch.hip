@@ -487,11 +487,7 @@ int QPHIX_D3_asqtad_invert_multi_rhs(QPHIX_info_t *info, QPHIX_D3_FermionLinksAs
   }
   QPHIX_REQUIRE(rhs_block == 0 || rhs_block == 1 || rhs_block == 2 || rhs_block == 4 || rhs_block == 8,
                 "asqtad_invert_multi_rhs: rhs_block must be 0 (autotune), 1, 2, 4 or 8", 0);
-  if (!L->sloppy) {
-    L->sloppy = new LinksT<float>;
-    links_to_float(*L, *L->sloppy);
-  }
-  const MrhsTune &tune = autotune_mrhs(*L->sloppy);
+  const MrhsTune &tune = autotune_mrhs(sloppy_links(L));
   if (tune.nr == 0) return 0;                 // workspace allocation failed (error recorded)
   int nb = rhs_block > 0 ? rhs_block : tune.nr;
   while (nb > 1 && nb / 2 >= nrhs) nb /= 2;   // no block wider than the sources need

@@ -178,6 +178,12 @@ def dslash_time(L, out, inp, parity, reps):
     return L.s.fn(L.prec, "asqtad_dslash_time")(L.h, out.h, inp.h, parity, reps)
 
 
+def dslash_sloppy(L, out, inp, parity, link_bits, reps=1):
+    """out = D inp with the FP64 links' sloppy copy (link_bits 32: FP32,
+    16: 16-bit fixed point) on FP32 vectors; milliseconds per application."""
+    return L.s.fn("D", "asqtad_dslash_sloppy")(L.h, out.h, inp.h, parity, link_bits, reps)
+
+
 def invert(L, out, inp, mass, parity, resid, relresid=0.0, niter=1000, nrestart=5):
     info = Info()
     inv = InvertArg(parity, niter, 0, nrestart, parity)

@@ -1,0 +1,84 @@
+"""16-bit fixed-point sloppy links: the Dslash they give against the FP64
+oracle (the quantisation bound: per element at most s / 65534 of its link's
+largest element s), and mixed-precision CG with them reaching the FP64
+system's true residual."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import lattice_session
+from helpers import half, random_links, rel_l2
+from pyqphix import api as A
+from pyqphix import fields as F
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("recon", [0, 18])
+def test_sloppy_dslash_against_oracle(recon):
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    raw = [F.to_raw(a) for a in random_links(lat, "weak", 1013)]
+    orc = O.Oracle(dims, *raw)
+    s.lib.QPHIX_MI355X_set_long_recon(recon)
+    try:
+        L = A.Links(s, *raw)
+    finally:
+        s.lib.QPHIX_MI355X_set_long_recon(0)
+    assert s.fn("D", "asqtad_links_recon")(L.h) == (12 if recon == 0 else 18)
+    src = F.to_raw(F.gaussian_source(lat, 7), np.float32)
+    vin = A.Vector(s, src, A.EVENODD, "F")
+    vout = A.Vector(s, None, A.EVENODD, "F")
+    for parity, cb in ((A.EVEN, 0), (A.ODD, 1)):
+        ref = orc.dslash(half(src.astype(np.float64), lat, 1 - cb), cb)
+        err = {}
+        for bits in (32, 16):
+            print(f"[case] sloppy dslash recon={recon} parity={parity} bits={bits}", flush=True)
+            assert A.dslash_sloppy(L, vout, vin, parity, bits) > 0
+            err[bits] = rel_l2(half(vout.extract(), lat, cb), ref)
+        assert err[32] < 1e-6
+        # uniform rounding of 16-bit elements: ~1e-5 relative; the 16-bit path was taken
+        assert 1e-7 < err[16] < 6e-5
+    for o in (L, vin, vout):
+        o.free()
+
+
+def true_rsq(orc, lat, x, b, mass, cb):
+    xh = half(x, lat, cb)
+    bh = half(b, lat, cb)
+    r = bh - (4 * mass * mass * xh - orc.dslash(orc.dslash(xh, 1 - cb), cb))
+    return float(r @ r / (bh @ bh))
+
+
+@pytest.mark.parametrize("bits", [32, 16])
+@pytest.mark.parametrize("kind,mass", [("haar", 0.025), ("weak", 0.01)])
+def test_mixed_with_16bit_links(bits, kind, mass):
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    raw = [F.to_raw(a) for a in random_links(lat, kind, 1001)]
+    orc = O.Oracle(dims, *raw)
+    L = A.Links(s, *raw)
+    b = F.to_raw(F.gaussian_source(lat, 93, parity=0))
+    vb = A.Vector(s, b, A.EVEN)
+    vx = A.Vector(s, None, A.EVEN)
+    resid = 1e-20
+    print(f"[case] mixed 16-bit links bits={bits} kind={kind}", flush=True)
+    s.lib.QPHIX_MI355X_set_sloppy_link_bits(16)
+    try:
+        ret, info, res = A.invert_mixed(L, vx, vb, mass, A.EVEN, resid, sloppy_bits=bits,
+                                        niter=3000, nrestart=10)
+    finally:
+        s.lib.QPHIX_MI355X_set_sloppy_link_bits(32)
+    x = vx.extract()
+    assert info.status == 0
+    assert res.final_rsq < resid
+    assert true_rsq(orc, lat, x, b, mass, 0) < 2 * resid
+    xo, io = orc.invert(b, mass, O.EVEN, resid)
+    assert rel_l2(half(x, lat, 0), half(xo, lat, 0)) < 1e-7
+    print(f"[case] iterations {res.final_iter} (FP64 oracle {io['final_iter']}), updates {res.final_restart}",
+          flush=True)
+    assert res.final_iter < 2.5 * io["final_iter"]
+    for o in (L, vb, vx):
+        o.free()
