@@ -34,9 +34,11 @@ struct GFGeo {
   int nxh, vh;
 };
 
-// Steps of the 48 loop walks through one link: 6 plaquettes x 3 steps,
-// 18 rectangles x 5, 24 parallelograms x 5.
-constexpr int GF_MAX_STEPS = 6 * 3 + 18 * 5 + 24 * 5;
+// Steps of the 48 loop walks through one link (6 plaquettes, 18 rectangles,
+// 24 parallelograms), grouped by their last link (see gf_step_tables): 6
+// groups of 7 walks with 2 + 4 + 4 + 4 x 4 steps and a closing step, 1 group
+// of 6 walks with 6 x 4 steps and a closing step, 1 padding step (even count).
+constexpr int GF_MAX_STEPS = 6 * (2 + 4 + 4 + 16 + 1) + (6 * 4 + 1) + 1;
 // Padding steps = the largest prefetch distance D compiled (distance 2
 // measured the same as 1 at 24^4, with 17 more registers).
 constexpr int GF_PAD = 2;
@@ -51,14 +53,14 @@ template <typename T> struct GFArgs {
   double eps;
   int nsteps;       // steps per direction (walks of zero coefficient left out)
   int order;        // block order, see gf_block
-  // per direction mu, one byte per step (see step_link), four to a word: the
+  // per direction mu, 16 bits per step (see step_link), two to a word: the
   // kernel reads them with uniform indices, as scalar dword loads
   // (GF_PAD padding steps after the last: the pipeline's final prefetches load them)
-  uint32_t tab[4][(GF_MAX_STEPS + GF_PAD + 3) / 4];
+  uint32_t tab[4][(GF_MAX_STEPS + GF_PAD + 1) / 2];
 };
 
 template <typename T> __device__ __forceinline__ unsigned gf_step(const GFArgs<T> &a, int mu, int k) {
-  return (a.tab[mu][k >> 2] >> (8 * (k & 3))) & 0xffu;
+  return (a.tab[mu][k >> 1] >> (16 * (k & 1))) & 0xffffu;
 }
 
 struct DM3 {
@@ -122,50 +124,55 @@ __device__ __forceinline__ void walker_move(Walker &w, int d, int sgn, const GFG
   }
 }
 
-// Link at the walker's vertex in direction d, of parity p (uniform: every lane
-// of the wavefront is at a vertex of the same parity).  dag = false: U_d;
-// dag = true: the transpose of U_d (the elements of U_d^dag with their
-// imaginary parts not yet negated; the step applies that sign when the link is
-// used, so nothing consumes the loaded values early and the loads stay in
-// flight).  Each element is one load from a uniform base (scalar) plus the
-// lane's site offset.  NE = 6: rows 0-1 only are stored; the elements of row 2
-// of U (row 2 of m, or column 2 when dag) are left for recon_row2.
+// Link U_d at the walker's vertex, of parity p (uniform: every lane of the
+// wavefront is at a vertex of the same parity), elements in storage order
+// (m.re/im[3 row + column]); NE = 6: rows 0-1 only (row 2 is rebuilt by
+// recon_row2 when the link is used).  Each element is one load from a uniform
+// (scalar) base plus the lane's site offset.  Nothing here depends on whether
+// the step uses the link daggered: the loads are the same either way, the
+// values are not touched until the step applies them (so the loads stay in
+// flight), and the registers they land in are fixed (a dag-dependent placement
+// was compiled into indexed private memory).
 template <int NE, typename T>
-__device__ __forceinline__ void load_link_t(const GFArgs<T> &a, const Walker &w, int d, int p, bool dag,
-                                            DM3 &m) {
+__device__ __forceinline__ void load_link_t(const GFArgs<T> &a, const Walker &w, int d, int p, DM3 &m) {
   using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
   const T2 *base = reinterpret_cast<const T2 *>(a.w0) + (ptrdiff_t)p * a.pdiff +
                    (size_t)(d * NE) * (size_t)a.g.vh;
   const unsigned i = (unsigned)w.r >> 1;
 #pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-      if (NE == 6 && (dag ? cc : r) == 2) continue;
-      const int src = dag ? 3 * cc + r : 3 * r + cc;
-      const T2 v = (base + (size_t)src * (size_t)a.g.vh)[i];
-      m.re[3 * r + cc] = (double)v.x;
-      m.im[3 * r + cc] = (double)v.y;
-    }
+  for (int e = 0; e < NE; ++e) {
+    const T2 v = (base + (size_t)e * (size_t)a.g.vh)[i];
+    m.re[e] = (double)v.x;
+    m.im[e] = (double)v.y;
+  }
 }
 
-// Row 2 of U from rows 0-1: U2 = conj(U0 x U1), on m as load_link_t<6>
-// leaves it (raw elements: m = U, or m = U^T when dag).
-template <bool DAG> __device__ __forceinline__ void recon_row2_t(DM3 &m) {
-  // element (row a, column b) of U in m (compile-time indices keep m in registers)
-  auto at = [](int a, int b) { return DAG ? 3 * b + a : 3 * a + b; };
+// Row 2 of U from rows 0-1: U2 = conj(U0 x U1).
+__device__ __forceinline__ void recon_row2(DM3 &m) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-    const double ar = m.re[at(0, c1)], ai = m.im[at(0, c1)], br = m.re[at(1, c2)], bi = m.im[at(1, c2)];
-    const double er = m.re[at(0, c2)], ei = m.im[at(0, c2)], fr = m.re[at(1, c1)], fi = m.im[at(1, c1)];
-    m.re[at(2, c)] = (ar * br - ai * bi) - (er * fr - ei * fi);
-    m.im[at(2, c)] = -((ar * bi + ai * br) - (er * fi + ei * fr));
+    const double ar = m.re[c1], ai = m.im[c1], br = m.re[3 + c2], bi = m.im[3 + c2];
+    const double er = m.re[c2], ei = m.im[c2], fr = m.re[3 + c1], fi = m.im[3 + c1];
+    m.re[6 + c] = (ar * br - ai * bi) - (er * fr - ei * fi);
+    m.im[6 + c] = -((ar * bi + ai * br) - (er * fi + ei * fr));
   }
 }
-__device__ __forceinline__ void recon_row2(DM3 &m, bool dag) {
-  if (dag) recon_row2_t<true>(m);
-  else recon_row2_t<false>(m);
+
+// m <- m^dag in place.
+__device__ __forceinline__ void dm_dagger(DM3 &m) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    m.im[4 * r] = -m.im[4 * r];
+#pragma unroll
+    for (int c = r + 1; c < 3; ++c) {
+      const double tr = m.re[3 * r + c], ti = m.im[3 * r + c];
+      m.re[3 * r + c] = m.re[3 * c + r];
+      m.im[3 * r + c] = -m.im[3 * c + r];
+      m.re[3 * c + r] = tr;
+      m.im[3 * c + r] = -ti;
+    }
+  }
 }
 
 // r <- r * m', m' = m (CONJ false) or m with its imaginary parts negated
@@ -197,26 +204,28 @@ template <bool CONJ> __device__ __forceinline__ void dm_rmul_t(DM3 &r, const DM3
   }
 }
 
-// One step of a loop walk, as a byte of the per-direction step table:
-// bits 0-1 direction d, bit 2 backward, bit 3 first step of a walk (the walk
-// starts at x + mu), bit 4 last step (the walk's product is added to the
-// staple sum), bits 5-6 index of the walk's coefficient, bit 7 parity of the
-// step's link relative to x (the number of unit moves from x to the link's
-// site, mod 2).
-enum : unsigned { GF_BACK = 4, GF_FIRST = 8, GF_LAST = 16, GF_PODD = 128 };
+// One step, as 16 bits of the per-direction step table: bits 0-1 direction
+// d; GF_BACK: the walker moves back along d before the load (else forward
+// after it); GF_DAG: the link is used daggered; GF_FIRST: first step of a walk
+// (the walker starts at x + mu; the product restarts at this link); GF_ATX: the
+// walker starts at x; GF_LAST: the walk's product, times its coefficient, is
+// added to the staple sum S; GF_GACC: ... added to the group sum G instead;
+// GF_GFIN: closing step of a group: S += G * link, G = 0; bits 5-6 the walk's
+// coefficient; GF_PODD: parity of the link's site relative to x (unit moves
+// from x, mod 2).  A backward step of a walk is GF_BACK | GF_DAG.
+enum : unsigned {
+  GF_BACK = 4, GF_FIRST = 8, GF_LAST = 16, GF_PODD = 128,
+  GF_GACC = 256, GF_GFIN = 512, GF_ATX = 1024, GF_DAG = 2048
+};
 
 // Link of one step from the walker's vertex (x's parity cb), as load_link_t
 // leaves it; the walker moves to the step's end.
 template <int NE, typename T>
 __device__ __forceinline__ void step_link(const GFArgs<T> &a, Walker &w, unsigned e, int cb, DM3 &m) {
   const int d = e & 3, p = cb ^ ((e & GF_PODD) ? 1 : 0);
-  if (e & GF_BACK) {
-    walker_move(w, d, -1, a.g);
-    load_link_t<NE>(a, w, d, p, true, m);
-  } else {
-    load_link_t<NE>(a, w, d, p, false, m);
-    walker_move(w, d, 1, a.g);
-  }
+  if (e & GF_BACK) walker_move(w, d, -1, a.g);
+  load_link_t<NE>(a, w, d, p, m);
+  if (!(e & GF_BACK)) walker_move(w, d, 1, a.g);
 }
 
 // The force's own link U_mu(x), all 3 rows, x at index i of parity cb.
@@ -277,13 +286,15 @@ __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFA
     x[3] = r / a.g.L[2];
     x[0] = 2 * xh + ((x[1] + x[2] + x[3] + cb) & 1);
   }
-  // every rest path starts at x + mu
-  Walker y{{x[0], x[1], x[2], x[3]}, x[0] + a.g.L[0] * (x[1] + a.g.L[1] * (x[2] + a.g.L[2] * x[3]))};
+  // the walks start at x + mu (group closing steps at x)
+  const Walker xw{{x[0], x[1], x[2], x[3]}, x[0] + a.g.L[0] * (x[1] + a.g.L[1] * (x[2] + a.g.L[2] * x[3]))};
+  Walker y = xw;
   walker_move(y, mu, 1, a.g);
-  __shared__ double sacc[18 * GF_BLOCK];
-  double *S = sacc + threadIdx.x;
+  // staple sum S and group sum G, per lane in LDS ([element][lane])
+  __shared__ double sacc[36 * GF_BLOCK];
+  double *S = sacc + threadIdx.x, *G = sacc + 18 * GF_BLOCK + threadIdx.x;
 #pragma unroll
-  for (int e = 0; e < 18; ++e) S[e * GF_BLOCK] = 0;
+  for (int e = 0; e < 36; ++e) S[e * GF_BLOCK] = 0;
   // All 48 loop walks through (x, mu) as one software-pipelined stream of
   // steps: the link of step k + D (of this walk or a later one) is requested
   // before the product with step k's link, so each load's latency overlaps D
@@ -298,32 +309,39 @@ __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFA
   auto fetch = [&](int k, Walker &w, DM3 &buf) -> unsigned {
     const unsigned e = gf_step(a, mu, k);
     if (e & GF_FIRST) w = y;
+    if (e & GF_ATX) w = xw;
     step_link<NE>(a, w, e, cb, buf);
     return e;
   };
   // apply a step's link (table byte e) to the walk's running product r
   auto apply = [&](unsigned e, DM3 &r, DM3 &cur) {
-    if constexpr (NE == 6) recon_row2(cur, (e & GF_BACK) != 0);
+    if constexpr (NE == 6) recon_row2(cur);
+    if (e & GF_DAG) dm_dagger(cur);  // uniform branch
     if (e & GF_FIRST) {
-      const double sg = (e & GF_BACK) ? -1.0 : 1.0;
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
         r.re[q] = cur.re[q];
-        r.im[q] = sg * cur.im[q];
+        r.im[q] = cur.im[q];
       }
     } else {
-      if (e & GF_BACK) {  // uniform branch: conjugate the link in place
+      if (e & GF_GFIN) {  // the group's shared last link: r = G (then G = 0)
 #pragma unroll
-        for (int q = 0; q < 9; ++q) cur.im[q] = -cur.im[q];
+        for (int q = 0; q < 9; ++q) {
+          r.re[q] = G[(2 * q) * GF_BLOCK];
+          r.im[q] = G[(2 * q + 1) * GF_BLOCK];
+          G[(2 * q) * GF_BLOCK] = 0;
+          G[(2 * q + 1) * GF_BLOCK] = 0;
+        }
       }
       dm_rmul_t<false>(r, cur);
     }
-    if (e & GF_LAST) {
-      const double c = a.c[(e >> 5) & 3];
+    if (e & (GF_LAST | GF_GACC | GF_GFIN)) {
+      const double c = (e & GF_GFIN) ? 1.0 : a.c[(e >> 5) & 3];
+      double *acc = (e & GF_GACC) ? G : S;
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        S[(2 * q) * GF_BLOCK] += c * r.re[q];
-        S[(2 * q + 1) * GF_BLOCK] += c * r.im[q];
+        acc[(2 * q) * GF_BLOCK] += c * r.re[q];
+        acc[(2 * q + 1) * GF_BLOCK] += c * r.im[q];
       }
     }
   };
@@ -372,66 +390,93 @@ __global__ void __launch_bounds__(GF_BLOCK) QPHIX_GF_OCC k_gauge_force(const GFA
 
 // Step tables of the loop walks through (x, mu), for each mu.  A walk is a
 // list of signed steps (+(d+1) forward along d, -(d+1) backward) from x + mu
-// back to x; for the link's own direction m = mu + 1 and another direction n:
+// back to x; for the link's own direction m = mu + 1 and another direction
+// n = +-(nu+1):
 //   plaquette   n, -m, -n
 //   rectangles  m, n, -m, -m, -n  /  n, -m, -m, -n, m  /  n, n, -m, -n, -n
-// for n = +-(nu+1), and for two other directions A, B (ordered, signed):
+// and for two other directions A, B (ordered, signed):
 //   parallelogram  A, B, -m, -A, -B.
-// Walks whose coefficient is zero are left out.
+// Walks that end with the same link share its product: for each n, the 7
+// walks ending with -n (the plaquette, the first and third rectangles, the 4
+// parallelograms with B = n) add their products without that last step,
+// times their coefficients, to G, and a closing step adds G times the shared
+// link (U_n(x)^dag for n > 0, U_|n|(x - |n|) for n < 0) to S; likewise the 6
+// second rectangles, which end with +m (link U_m(x - m)).  180 link products
+// become 139.  Walks of a zero coefficient are left out, and so is the
+// closing step of an empty group.
 template <typename T> static bool gf_step_tables(GFArgs<T> &a) {
   std::memset(a.tab, 0, sizeof(a.tab));
-  int n = 0;
+  int n = -1;
   for (int mu = 0; mu < 4; ++mu) {
     int k = 0;
+    auto put = [&](unsigned e) {
+      a.tab[mu][k >> 1] |= e << (16 * (k & 1));
+      ++k;
+    };
+    // a walk without its last step; the product of the kept steps goes to G
     auto walk = [&](std::initializer_list<int> steps, unsigned ci) {
-      const int len = (int)steps.size();
+      const int kept = (int)steps.size() - 1;
       int j = 0;
       for (int st : steps) {
+        if (j == kept) break;
         unsigned e = (unsigned)((st > 0 ? st : -st) - 1) | (ci << 5);
-        if (st < 0) e |= GF_BACK;
+        if (st < 0) e |= GF_BACK | GF_DAG;
         // unit moves from x to the link's site: 1 (to x + mu) + j, + 1 more
         // for a backward step (its link sits at the step's end)
         if ((1 + j + (st < 0 ? 1 : 0)) & 1) e |= GF_PODD;
         if (j == 0) e |= GF_FIRST;
-        if (j == len - 1) e |= GF_LAST;
-        a.tab[mu][k >> 2] |= e << (8 * (k & 3));
-        ++k;
+        if (j == kept - 1) e |= GF_GACC;
+        put(e);
         ++j;
       }
+    };
+    // closing step of a group whose walks end with step `last` into x
+    auto close = [&](int last) {
+      unsigned e = (unsigned)((last > 0 ? last : -last) - 1) | GF_ATX | GF_GFIN;
+      if (last > 0) e |= GF_BACK | GF_PODD;  // link at x - d (one move from x), undaggered
+      else e |= GF_DAG;                       // link at x, daggered
+      put(e);
     };
     const int m = mu + 1;
     for (int nu = 0; nu < 4; ++nu) {
       if (nu == mu) continue;
       for (int sg = -1; sg <= 1; sg += 2) {
         const int d = sg * (nu + 1);
+        const int k0 = k;
         if (a.c[0] != 0) walk({d, -m, -d}, 0);
         if (a.c[1] != 0) {
           walk({m, d, -m, -m, -d}, 1);
-          walk({d, -m, -m, -d, m}, 1);
           walk({d, d, -m, -d, -d}, 1);
         }
+        if (a.c[2] != 0)
+          for (int da = 0; da < 4; ++da) {
+            if (da == mu || da == nu) continue;
+            for (int sa = -1; sa <= 1; sa += 2) {
+              const int A = sa * (da + 1);
+              walk({A, d, -m, -A, -d}, 2);
+            }
+          }
+        if (k > k0) close(-d);
       }
     }
-    if (a.c[2] != 0) {
-      for (int da = 0; da < 4; ++da) {
-        if (da == mu) continue;
-        for (int db = 0; db < 4; ++db) {
-          if (db == mu || db == da) continue;
-          for (int sa = -1; sa <= 1; sa += 2)
-            for (int sb = -1; sb <= 1; sb += 2) {
-              const int A = sa * (da + 1), B = sb * (db + 1);
-              walk({A, B, -m, -A, -B}, 2);
-            }
+    if (a.c[1] != 0) {
+      for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
+        for (int sg = -1; sg <= 1; sg += 2) {
+          const int d = sg * (nu + 1);
+          walk({d, -m, -m, -d, m}, 1);
         }
       }
+      close(m);
     }
-    // padding steps: forward steps from x + mu (one move from x) along
-    // direction 0, loaded by the last prefetches and never applied
-    for (int j = 0; j < GF_PAD; ++j)
-      a.tab[mu][(k + j) >> 2] |= (GF_FIRST | GF_PODD) << (8 * ((k + j) & 3));
+    // an even count (two steps per pipeline trip): a one-step walk that adds
+    // nothing; then the padding steps the last prefetches load, never applied
+    if (k & 1) put(GF_FIRST | GF_PODD);
+    QPHIX_REQUIRE(n < 0 || n == k, "gauge force: step tables differ between directions", false);
     n = k;
+    for (int j = 0; j < GF_PAD; ++j) put(GF_FIRST | GF_PODD);
   }
-  QPHIX_REQUIRE(n % 6 == 0 && n <= GF_MAX_STEPS, "gauge force: bad step table", false);
+  QPHIX_REQUIRE(n % 2 == 0 && n <= GF_MAX_STEPS, "gauge force: bad step table", false);
   a.nsteps = n;
   return true;
 }
