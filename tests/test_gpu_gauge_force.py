@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 MILC_B6 = F.symanzik_coeffs(6.0, 0.86)   # beta = 6.0, u0 = 0.86
 
 
-@pytest.mark.parametrize("coef", [MILC_B6, (1.0, 0.0, 0.0), (0.3, -0.2, 0.1)])
+# (zero coefficients drop walks, groups and their closing steps from the step tables)
+@pytest.mark.parametrize("coef", [MILC_B6, (1.0, 0.0, 0.0), (0.3, -0.2, 0.1), (0.0, -0.2, 0.0), (0.0, 0.0, 0.1)])
 def test_gauge_force_matches_oracle(coef):
     dims = (8, 8, 8, 8)
     s = lattice_session(dims)
