@@ -297,30 +297,44 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds(const MrhsArgs
       wave_lds_sync();
       if (mu < 3) fetch(mu + 1, pre);
       if (active) {
-        cplx<T> u[18];
+        // one link at a time (fewer live registers than a whole pair); complex
+        // k of a pair starting at chunk c0 sits in LDS chunk c0 + k / CPV.
+        // Same accumulation order as apply_links_k: forward, then back.
+        auto lds_c = [&](int c0, int k) -> cplx<T> {
+          const V d = wl[(c0 + k / CPV) * SW + s];
+          if constexpr (CPV == 2) return (k % 2) ? cplx<T>{d.z, d.w} : cplx<T>{d.x, d.y};
+          else return cplx<T>{d.x, d.y};
+        };
+        cplx<T> w[9], v[3];
 #pragma unroll
-        for (int k = 0; k < NCH_F; ++k) {
-          const V d = wl[k * SW + s];
-          if constexpr (CPV == 2) {
-            u[2 * k] = cplx<T>{d.x, d.y};
-            u[2 * k + 1] = cplx<T>{d.z, d.w};
-          } else {
-            u[k] = cplx<T>{d.x, d.y};
-          }
-        }
-        apply_links_k<T, 18, NR, 1>(u, a.in, n.f1[mu], n.b1[mu], r0, 0, 0, acc);
+        for (int e = 0; e < 9; ++e) w[e] = lds_c(0, e);
+        Spin<T>::load(a.in, n.f1[mu] * NR + r0, v);
+        matvec_acc(w, v, acc[0]);
 #pragma unroll
-        for (int k = 0; k < NCH_L; ++k) {
-          const V d = wl[(NCH_F + k) * SW + s];
-          if constexpr (CPV == 2) {
-            u[2 * k] = cplx<T>{d.x, d.y};
-            u[2 * k + 1] = cplx<T>{d.z, d.w};
-          } else {
-            u[k] = cplx<T>{d.x, d.y};
-          }
+        for (int e = 0; e < 9; ++e) w[e] = lds_c(0, 9 + e);
+        Spin<T>::load(a.in, n.b1[mu] * NR + r0, v);
+        matvec_acc(w, v, acc[0]);
+        if constexpr (NL == 18) {
+#pragma unroll
+          for (int e = 0; e < 9; ++e) w[e] = lds_c(NCH_F, e);
+          Spin<T>::load(a.in, n.f3[mu] * NR + r0, v);
+          matvec_acc(w, v, acc[0]);
+#pragma unroll
+          for (int e = 0; e < 9; ++e) w[e] = lds_c(NCH_F, 9 + e);
+          Spin<T>::load(a.in, n.b3[mu] * NR + r0, v);
+          matvec_acc(w, v, acc[0]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 6; ++e) w[e] = lds_c(NCH_F, e);
+          recon_row2(w, ((sg >> (2 * mu + 1)) & 1u) ? (T)-1 : (T)1);
+          Spin<T>::load(a.in, n.f3[mu] * NR + r0, v);
+          matvec_acc(w, v, acc[0]);
+#pragma unroll
+          for (int e = 0; e < 6; ++e) w[e] = lds_c(NCH_F, 6 + e);
+          recon_row2(w, ((sg >> (2 * mu)) & 1u) ? (T)-1 : (T)1);
+          Spin<T>::load(a.in, n.b3[mu] * NR + r0, v);
+          matvec_acc(w, v, acc[0]);
         }
-        apply_links_k<T, NL, NR, 1>(u, a.in, n.f3[mu], n.b3[mu], r0, (sg >> (2 * mu + 1)) & 1u,
-                                    (sg >> (2 * mu)) & 1u, acc);
       }
     }
     if (active) {
