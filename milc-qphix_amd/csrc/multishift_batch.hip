@@ -90,11 +90,30 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   if (threadIdx.x < NR) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sum;
 }
 
-// x_j += beta_j p_j ; p_j = zeta_j r + alpha_j p_j  for every shift j < num_now,
-// one work-item per (site, source); layout [shift][site][source][6].
+// One step of shift j: x += beta p ; p = zeta r + alpha p (explicit fmas: the
+// one-step and two-step paths below round identically).
+template <typename T>
+__device__ __forceinline__ void msx_one(cplx<T> *xv, cplx<T> *pv, const cplx<T> *rv, T bj, T aj, T zj) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    xv[c].re = fma(bj, pv[c].re, xv[c].re);
+    xv[c].im = fma(bj, pv[c].im, xv[c].im);
+    pv[c].re = fma(zj, rv[c].re, aj * pv[c].re);
+    pv[c].im = fma(zj, rv[c].im, aj * pv[c].im);
+  }
+}
+
+// x_j += beta_j p_j ; p_j = zeta_j r + alpha_j p_j  for every shift j < num_now
+// not yet retired, one work-item per (site, source); layout
+// [shift][site][source][6].  Only the base system's p feeds the operator, so
+// the shifted systems go two steps at a time: on a deferred step (st.defer)
+// only the base system is updated and r is saved in rprev; on the next step
+// (st.pend) each shifted system does both steps in registers, reading and
+// writing x_j and p_j once for two steps (same arithmetic and order as two
+// single steps).
 template <typename T, int NR>
 __global__ void __launch_bounds__(BLAS_BLOCK)
-    k_msx_update_pm(T *xs, T *p, const T *rs, int vh, int nshift, const SolverState *st) {
+    k_msx_update_pm(T *xs, T *p, const T *rs, T *rprev, int vh, int nshift, const SolverState *st) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= vh * NR) return;
   const int r = t % NR;
@@ -103,20 +122,42 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   const size_t stride = (size_t)vh * NR;
   cplx<T> rv[3];
   Spin<T>::load(rs, t, rv);
-  const int nn = s.num_now;
-  for (int j = 0; j < nn && j < nshift; ++j) {
-    if (s.sdone[j] == 1) continue;
-    const T bj = (T)s.beta_i[j], aj = (T)s.alpha[j], zj = (T)s.zeta_ip1[j];
+  const int nn = s.num_now, jl = s.j_low;
+  auto step1 = [&](int j) {
     cplx<T> xv[3], pv[3];
     Spin<T>::load(xs + 6 * stride * j, t, xv);
     Spin<T>::load(p + 6 * stride * j, t, pv);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      xv[c].re += bj * pv[c].re;
-      xv[c].im += bj * pv[c].im;
-      pv[c].re = zj * rv[c].re + aj * pv[c].re;
-      pv[c].im = zj * rv[c].im + aj * pv[c].im;
+    msx_one<T>(xv, pv, rv, (T)s.beta_i[j], (T)s.alpha[j], (T)s.zeta_ip1[j]);
+    Spin<T>::store(xs + 6 * stride * j, t, xv);
+    Spin<T>::store(p + 6 * stride * j, t, pv);
+  };
+  if (s.defer) {
+    Spin<T>::store(rprev, t, rv);
+    step1(jl);
+    return;
+  }
+  if (!s.pend) {
+    for (int j = 0; j < nn && j < nshift; ++j) {
+      if (s.sdone[j] == 1) continue;
+      step1(j);
     }
+    return;
+  }
+  cplx<T> r1[3];
+  Spin<T>::load(rprev, t, r1);
+  // num_now only shrinks, so the pending step covers every shift of this one
+  const int na = s.pend_nn;
+  for (int j = 0; j < na && j < nshift; ++j) {
+    if (s.sdone[j] == 1) continue;  // retired before the pending step
+    if (j == jl) {
+      step1(j);
+      continue;
+    }
+    cplx<T> xv[3], pv[3];
+    Spin<T>::load(xs + 6 * stride * j, t, xv);
+    Spin<T>::load(p + 6 * stride * j, t, pv);
+    msx_one<T>(xv, pv, r1, (T)s.pbeta[j], (T)s.palpha[j], (T)s.pzeta[j]);
+    if (j < nn) msx_one<T>(xv, pv, rv, (T)s.beta_i[j], (T)s.alpha[j], (T)s.zeta_ip1[j]);
     Spin<T>::store(xs + 6 * stride * j, t, xv);
     Spin<T>::store(p + 6 * stride * j, t, pv);
   }
@@ -127,9 +168,22 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
 __global__ void k_msx_commit(SolverState *st, int nr) {
   const int r = threadIdx.x;
   if (r >= nr || !st[r].active) return;
-  for (int j = 0; j < st[r].num_now; ++j)
-    if (st[r].sdone[j] == 2) st[r].sdone[j] = 1;
-  if (st[r].ru) st[r].active = 0;
+  SolverState &s = st[r];
+  for (int j = 0; j < s.num_now; ++j)
+    if (s.sdone[j] == 2) s.sdone[j] = 1;
+  if (s.defer) {  // the shifted updates of this step wait for the next one
+    s.pend = 1;
+    s.pend_nn = s.num_now;
+    for (int j = 0; j < s.num_now; ++j) {
+      s.pbeta[j] = s.beta_i[j];
+      s.palpha[j] = s.alpha[j];
+      s.pzeta[j] = s.zeta_ip1[j];
+    }
+  } else {
+    s.pend = 0;
+  }
+  s.defer = 0;
+  if (s.ru) s.active = 0;
 }
 
 // Source slot r: r_s = p_j = (T) b for all j, x_s = 0.
@@ -218,6 +272,9 @@ inline int mrhs_nlog(int vh, int NR, int K) {
 // link kernel (k_dslash_mrhs_lds; bit-identical results) unless switched off
 // (QPHIX_MI355X_set_mrhs_lds(0), or QPHIX_MRHS_LDS=0 before the first call).
 static int g_mrhs_lds = -1;
+// Two-step updates of the shifted systems in the batched multi-shift solver
+// (k_msx_update_pm; identical results): on unless switched off.
+static int g_msx_fuse = 1;
 static bool mrhs_lds_on() {
   if (g_mrhs_lds < 0) {
     const char *e = std::getenv("QPHIX_MRHS_LDS");
@@ -439,6 +496,7 @@ using namespace qphix;
 extern "C" {
 
 void QPHIX_MI355X_set_mrhs_lds(int on) { g_mrhs_lds = on ? 1 : 0; }
+void QPHIX_MI355X_set_msx_fuse(int on) { g_msx_fuse = on ? 1 : 0; }
 
 void QPHIX_F3_asqtad_dslash_multi(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVector *out[],
                                   QPHIX_F3_ColorVector *in[], int nvec, QPHIX_evenodd_t parity) {

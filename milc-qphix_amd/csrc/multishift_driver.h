@@ -16,7 +16,7 @@ struct BatchStats {
 // its gate.
 template <int NR, typename T>
 static void msx_batch(const LinksT<T> &Lk, int parity, int batch, T *p, T *xs, T *rs, T *Ap, T *tmp,
-                      int j_low, double offset_low, SolverState *st, int K, int nbd, int nbr,
+                      T *rprev, int j_low, double offset_low, SolverState *st, int K, int nbd, int nbr,
                       int nlog, int nshift) {
   const int vh = ctx().g.vh;
   const size_t nvr = 6 * (size_t)vh * NR;
@@ -31,7 +31,7 @@ static void msx_batch(const LinksT<T> &Lk, int parity, int batch, T *p, T *xs, T
     hipLaunchKernelGGL(k_finalize_multi<NR>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream,
                        ctx().partials, nbr, st, (int)SOP_MSX_RSQ2);
     hipLaunchKernelGGL((k_msx_update_pm<T, NR>), dim3((vh * NR + BLAS_BLOCK - 1) / BLAS_BLOCK), dim3(BLAS_BLOCK),
-                       0, ctx().stream, xs, p, rs, vh, nshift, st);
+                       0, ctx().stream, xs, p, rs, rprev, vh, nshift, st);
     hipLaunchKernelGGL(k_msx_commit, dim3(1), dim3(NR), 0, ctx().stream, st, NR);
   }
 }
@@ -55,12 +55,13 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
   const size_t nv = 6 * (size_t)vh, nvr = nv * NR;
   const LinksT<float> &Lf = *L->sloppy;
   const int other = 1 - parity;
-  // FP32 workspace: p, xs [nshift][vh][NR][6]; rs, Ap, tmp [vh][NR][6]; r64, tmp64; states; pointers
-  char *w = (char *)ws2_get((2 * (size_t)nshift + 3) * nvr * sizeof(float) + 2 * nv * sizeof(double) +
+  // FP32 workspace: p, xs [nshift][vh][NR][6]; rs, Ap, tmp, rprev [vh][NR][6]; r64, tmp64; states; pointers
+  char *w = (char *)ws2_get((2 * (size_t)nshift + 4) * nvr * sizeof(float) + 2 * nv * sizeof(double) +
                             (NR + 2) * sizeof(SolverState) + NR * nshift * sizeof(double *) + 4096);
   if (!w) return false;
   float *p = (float *)w, *xs = p + nshift * nvr, *rs = xs + nshift * nvr, *Ap = rs + nvr, *tmp = Ap + nvr;
-  double *r64 = (double *)(tmp + nvr), *tmp64 = r64 + nv;
+  float *rprev = tmp + nvr;
+  double *r64 = (double *)(rprev + nvr), *tmp64 = r64 + nv;
   SolverState *st = (SolverState *)(((uintptr_t)(tmp64 + nv) + 255) & ~(uintptr_t)255);
   SolverState *tst = st + NR;
   double **dx = (double **)(tst + 1);
@@ -68,12 +69,12 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
   bool hi = switch_frac > 0;
   double *w64 = nullptr;
   if (hi) {
-    w64 = (double *)dev_alloc((2 * (size_t)nshift + 3) * nvr * sizeof(double));
+    w64 = (double *)dev_alloc((2 * (size_t)nshift + 4) * nvr * sizeof(double));
     if (!w64) return false;
   }
   double *p_d = w64, *xs_d = hi ? p_d + nshift * nvr : nullptr;
   double *rs_d = hi ? xs_d + nshift * nvr : nullptr, *Ap_d = hi ? rs_d + nvr : nullptr;
-  double *tmp_d = hi ? Ap_d + nvr : nullptr;
+  double *tmp_d = hi ? Ap_d + nvr : nullptr, *rprev_d = hi ? tmp_d + nvr : nullptr;
   const int nb = (vh + BLAS_BLOCK - 1) / BLAS_BLOCK;
   const int nlog = (int)(((size_t)vh * NR + BLAS_BLOCK - 1) / BLAS_BLOCK);
   const int K64 = std::min(K, 2);  // FP64: twice the registers per source
@@ -137,6 +138,7 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     s.mx_delta = hi ? switch_frac : delta;
     s.max_cg = max_cg;
     s.active = 1;
+    s.fuse = (g_msx_fuse && nshift > 1) ? 1 : 0;  // one shift: nothing to defer
     if (hi)
       hipLaunchKernelGGL(k_msx_init<double>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs_d, p_d, xs_d,
                          b64[r], vh, NR, r, nshift);
@@ -157,10 +159,10 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
   const int batch = 16;
   auto enqueue_batch = [&]() {
     if (hi)
-      msx_batch<NR, double>(*L, parity, batch, p_d, xs_d, rs_d, Ap_d, tmp_d, j_low, offset_low, st, K64,
-                            nbd64, nbr64, nlog, nshift);
+      msx_batch<NR, double>(*L, parity, batch, p_d, xs_d, rs_d, Ap_d, tmp_d, rprev_d, j_low, offset_low, st,
+                            K64, nbd64, nbr64, nlog, nshift);
     else
-      msx_batch<NR, float>(Lf, parity, batch, p, xs, rs, Ap, tmp, j_low, offset_low, st, K, nbd, nbr,
+      msx_batch<NR, float>(Lf, parity, batch, p, xs, rs, Ap, tmp, rprev, j_low, offset_low, st, K, nbd, nbr,
                            nlog, nshift);
   };
   // grid queries (occupancy calculator) happen before any capture
@@ -216,6 +218,13 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     HIP_CHECK(hipMemcpyAsync(ctx().pinned, st, NR * sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
     HIP_CHECK(hipStreamSynchronize(ctx().stream));
     std::memcpy(hs.data(), ctx().pinned, NR * sizeof(SolverState));
+    // a paused or finished source never has a deferred update pending
+    for (int r = 0; r < nrhs; ++r)
+      if (!done[r] && !hs[r].active && hs[r].pend) {
+        dev_free(w64);
+        fail(__FILE__, __LINE__, "multi-RHS solver: a paused source has a pending shifted update");
+        return false;
+      }
     if (hi) {
       // FP64 phase: a source pauses once |r| < switch_frac |b| (or on
       // convergence or budget); the chunk drops to FP32 when all have.

@@ -32,6 +32,14 @@ struct SolverState {
   // update is its last), 1 retired; sdone_at = the step it converged at
   int sdone[MAXNDMASS];
   int sdone_at[MAXNDMASS];
+  // batched multi-shift, shifted systems updated two steps at a time: `defer`
+  // (set by this step's finalize) = the shifted updates of this step are left
+  // to the next one; `pend` (set by the commit) = such a step is pending, with
+  // its num_now and coefficients in pend_nn, pbeta, palpha, pzeta; `fuse` = 1
+  // allows deferring.  A step that requests a pause, or retires a shift, is
+  // never deferred, so a paused or finished source has nothing pending.
+  int defer, pend, pend_nn, fuse;
+  double pbeta[MAXNDMASS], palpha[MAXNDMASS], pzeta[MAXNDMASS];
 };
 
 enum ScalarOp {
@@ -184,12 +192,15 @@ __device__ inline void apply_scalar_op(SolverState *s, int op, const double *v) 
     if (s->rsq <= s->rsqstop || r < s->mx_delta * s->mx_maxr || s->iteration >= s->max_cg) s->ru = 1;
     // A shifted system's residual is zeta_{i+1,j} r: once that meets the
     // target the shift takes this step's update and then retires.
+    bool retire = false;
     for (int j = 0; j < s->num_now; ++j)
       if (j != s->j_low && s->sdone[j] == 0 && s->zeta_ip1[j] * s->zeta_ip1[j] * s->rsq < s->rsqstop) {
         s->sdone[j] = 2;
         s->sdone_at[j] = s->iteration;
+        retire = true;
       }
     ms_alpha_recurrence<double>(s);
+    s->defer = (s->fuse && !s->pend && !s->ru && !retire) ? 1 : 0;
     break;
   }
   case SOP_MX_RP:

@@ -167,6 +167,46 @@ def test_lds_staged_links_match_per_lane_loads():
         o.free()
 
 
+@pytest.mark.parametrize("block", [8, 4])
+def test_two_step_shifted_updates_match_single_steps(block):
+    """The batched solver's shifted systems updated two steps at a time (one
+    read and write of x_j, p_j per two steps) give the same solutions, bit for
+    bit, as one step at a time: through shift retirements, reliable-update
+    pauses, the FP64-to-FP32 switch and the per-shift refinement."""
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    raw = [F.to_raw(a) for a in random_links(lat, "weak", 1015)]
+    L = A.Links(s, *raw)
+    masses = [0.02, 0.05, 0.12, 0.3, 0.6]
+    bs = [F.to_raw(F.gaussian_source(lat, 540 + r, parity=0)) for r in range(6)]
+    bs[3][:] = 0.0
+    vb = [A.Vector(s, b, A.EVEN) for b in bs]
+    got = {}
+    try:
+        for fuse in (1, 0):
+            s.lib.QPHIX_MI355X_set_msx_fuse(fuse)
+            print(f"[case] two-step shifted updates fuse={fuse} block={block}", flush=True)
+            vx = [[A.Vector(s, None, A.EVEN) for _ in masses] for _ in bs]
+            ret, info, res = A.invert_multi_rhs(L, vx, vb, masses, A.EVEN, 1e-18, niter=1000,
+                                                nrestart=2, rhs_block=block)
+            assert info.status == 0
+            got[fuse] = (ret, [[x.extract() for x in row] for row in vx],
+                         [[(r.final_iter, r.final_rsq) for r in row] for row in res])
+            for row in vx:
+                for x in row:
+                    x.free()
+    finally:
+        s.lib.QPHIX_MI355X_set_msx_fuse(1)
+    assert got[1][0] == got[0][0]
+    assert got[1][2] == got[0][2]
+    for r1, r0 in zip(got[1][1], got[0][1]):
+        for x, y in zip(r1, r0):
+            np.testing.assert_array_equal(x, y)
+    for o in [L] + vb:
+        o.free()
+
+
 def test_multi_rhs_ragged_and_zero_source():
     """5 sources in a block of 8 (three empty slots), one of them zero: the
     zero source returns zero solutions, the others meet their targets."""
