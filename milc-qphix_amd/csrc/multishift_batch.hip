@@ -50,12 +50,17 @@ __global__ void __launch_bounds__(FINALIZE_BLOCK)
     s[r] = 0;
     for (int b = threadIdx.x; b < nb; b += FINALIZE_BLOCK) s[r] += partials[(size_t)r * nb + b];
   }
-  block_sum<NR>(s);
+  block_sum<NR>(s);  // totals in thread 0
+  // one thread per source runs its scalar recurrences (a chain of dependent
+  // state reads and writes each; in one thread for all sources they queued)
+  __shared__ double tot[NR];
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int r = 0; r < NR; ++r)
-      if (st[r].active) apply_scalar_op(st + r, op, s + r);
+    for (int r = 0; r < NR; ++r) tot[r] = s[r];
   }
+  __syncthreads();
+  const int r = threadIdx.x;
+  if (r < NR && st[r].active) apply_scalar_op(st + r, op, tot + r);
 }
 
 // r_r += beta_low,r Ap_r ; reduce |r_r|^2 per source.  One work-item per
