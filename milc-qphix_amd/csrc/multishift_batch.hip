@@ -116,6 +116,9 @@ __device__ __forceinline__ void msx_one(cplx<T> *xv, cplx<T> *pv, const cplx<T> 
 // (st.pend) each shifted system does both steps in registers, reading and
 // writing x_j and p_j once for two steps (same arithmetic and order as two
 // single steps).
+#ifndef MSX_GROUP
+#define MSX_GROUP 2
+#endif
 template <typename T, int NR>
 __global__ void __launch_bounds__(BLAS_BLOCK)
     k_msx_update_pm(T *xs, T *p, const T *rs, T *rprev, int vh, int nshift, const SolverState *st) {
@@ -141,30 +144,36 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     step1(jl);
     return;
   }
-  if (!s.pend) {
-    for (int j = 0; j < nn && j < nshift; ++j) {
-      if (s.sdone[j] == 1) continue;
-      step1(j);
-    }
-    return;
-  }
+  // Shifts MSX_GROUP at a time: the group's x_j, p_j loads are issued before
+  // any store, so a work-item's shifts are not one dependent load-store chain.
   cplx<T> r1[3];
-  Spin<T>::load(rprev, t, r1);
-  // num_now only shrinks, so the pending step covers every shift of this one
-  const int na = s.pend_nn;
-  for (int j = 0; j < na && j < nshift; ++j) {
-    if (s.sdone[j] == 1) continue;  // retired before the pending step
-    if (j == jl) {
-      step1(j);
-      continue;
+  const bool pend = s.pend != 0;
+  if (pend) Spin<T>::load(rprev, t, r1);
+  // num_now only shrinks, so a pending step covers every shift of this one
+  const int lim = min(pend ? s.pend_nn : nn, nshift);
+  for (int j0 = 0; j0 < lim; j0 += MSX_GROUP) {
+    bool on[MSX_GROUP];
+    cplx<T> xv[MSX_GROUP][3], pv[MSX_GROUP][3];
+#pragma unroll
+    for (int q = 0; q < MSX_GROUP; ++q) {
+      const int j = j0 + q;
+      on[q] = j < lim && s.sdone[j] != 1;  // retired earlier: no update
+      if (on[q]) {
+        Spin<T>::load(xs + 6 * stride * j, t, xv[q]);
+        Spin<T>::load(p + 6 * stride * j, t, pv[q]);
+      }
     }
-    cplx<T> xv[3], pv[3];
-    Spin<T>::load(xs + 6 * stride * j, t, xv);
-    Spin<T>::load(p + 6 * stride * j, t, pv);
-    msx_one<T>(xv, pv, r1, (T)s.pbeta[j], (T)s.palpha[j], (T)s.pzeta[j]);
-    if (j < nn) msx_one<T>(xv, pv, rv, (T)s.beta_i[j], (T)s.alpha[j], (T)s.zeta_ip1[j]);
-    Spin<T>::store(xs + 6 * stride * j, t, xv);
-    Spin<T>::store(p + 6 * stride * j, t, pv);
+#pragma unroll
+    for (int q = 0; q < MSX_GROUP; ++q) {
+      if (!on[q]) continue;
+      const int j = j0 + q;
+      if (pend && j != jl)  // the pending step, then this one
+        msx_one<T>(xv[q], pv[q], r1, (T)s.pbeta[j], (T)s.palpha[j], (T)s.pzeta[j]);
+      if (j < nn || j == jl)
+        msx_one<T>(xv[q], pv[q], rv, (T)s.beta_i[j], (T)s.alpha[j], (T)s.zeta_ip1[j]);
+      Spin<T>::store(xs + 6 * stride * j, t, xv[q]);
+      Spin<T>::store(p + 6 * stride * j, t, pv[q]);
+    }
   }
 }
 
