@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -36,8 +37,9 @@ template <typename T> struct OprodArgs {
   double c[OPROD_GROUP];
   int nt;                   // terms in this group (1..OPROD_GROUP)
   int accumulate;           // add to the outputs instead of overwriting them
-  double2 *o1, *o3;         // [cb][9][vh][mu]
+  double2 *o1, *o3;         // oprod_index layout
   Geo g;
+  SiteOrder order;          // traversal order of the sites
 };
 
 // Raw opposite-parity indices of y + mu and y + 3mu for site i of parity cb.
@@ -62,20 +64,47 @@ __device__ __forceinline__ void fwd_hops(const Geo &g, int i, int cb, int mu, in
 // Work-item t of parity cb handles site t / 4 and direction t % 4: the four
 // directions of a site sit in neighbouring lanes, so psi(y) is loaded once per
 // site from L1 and the four outputs of each element are one contiguous 64 B
-// store ([cb][element][site][mu] layout).
+// store.
+#ifndef QPHIX_OPROD_TILE
+#define QPHIX_OPROD_TILE 1
+#endif
+#if QPHIX_OPROD_TILE
+// Layout [cb][tile of 16 sites][element][site % 16][mu]: the 64 work-items of
+// 16 consecutive sites write one contiguous 9 KB record per output.
+constexpr int OPROD_TS = 16;
+__host__ __device__ __forceinline__ size_t oprod_tiles(int vh) { return (size_t)(vh + OPROD_TS - 1) / OPROD_TS; }
+__device__ __forceinline__ size_t oprod_index(int cb, int e, int i, int mu, int vh) {
+  return (((size_t)cb * oprod_tiles(vh) + i / OPROD_TS) * 9 + e) * (4 * OPROD_TS) + (i % OPROD_TS) * 4 + mu;
+}
+// double2 elements of one output (both parities)
+static inline size_t oprod_len(int vh) { return 2 * oprod_tiles(vh) * 9 * 4 * OPROD_TS; }
+#else
 __device__ __forceinline__ size_t oprod_index(int cb, int e, int i, int mu, int vh) {
   return ((size_t)(cb * 9 + e) * vh + i) * 4 + mu;
+}
+static inline size_t oprod_len(int vh) { return (size_t)72 * vh; }
+#endif
+
+// Outputs are written once and read much later (the next group or the export),
+// so they are stored non-temporally.
+__device__ __forceinline__ void oprod_store(double2 *o, size_t k, const cplx<double> &v) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(d2{v.re, v.im}, reinterpret_cast<d2 *>(o + k));
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_fermion_oprod(const OprodArgs<T> a) {
   const int vh = a.g.vh;
   // XCD x (blocks are dispatched round-robin over the 8 XCDs) sweeps the x-th
-  // contiguous slab of sites, so neighbour spinors are reused from its L2
+  // contiguous slab of t-slices; with a.order on it walks the slab one z-plane
+  // column at a time (every t of the slab, then the next plane), so the
+  // t + 1, t + 3 spinors a site gathers were loaded a few planes earlier and
+  // are still in that XCD's L2
   const int slot = block_slot(OPT_XCD);
   const int t = slot * blockDim.x + threadIdx.x;
-  const int i = t >> 2, mu = t & 3, cb = blockIdx.y;
-  if (i >= vh) return;
+  const int L = t >> 2, mu = t & 3, cb = blockIdx.y;
+  if (L >= vh) return;
+  const int i = ordered_site(a.g, a.order, L);
   int n1, n3;
   fwd_hops(a.g, i, cb, mu, n1, n3);
   cplx<double> o1[9], o3[9];
@@ -117,8 +146,8 @@ __global__ void __launch_bounds__(256) k_fermion_oprod(const OprodArgs<T> a) {
   }
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
-    a.o1[oprod_index(cb, e, i, mu, vh)] = make_double2(o1[e].re, o1[e].im);
-    a.o3[oprod_index(cb, e, i, mu, vh)] = make_double2(o3[e].re, o3[e].im);
+    oprod_store(a.o1, oprod_index(cb, e, i, mu, vh), o1[e]);
+    oprod_store(a.o3, oprod_index(cb, e, i, mu, vh), o3[e]);
   }
 }
 
@@ -133,6 +162,18 @@ __global__ void __launch_bounds__(256) k_oprod_export(const double2 *o, int cb, 
       raw[72 * (size_t)i + 18 * mu + 2 * e] = (T)v.x;
       raw[72 * (size_t)i + 18 * mu + 2 * e + 1] = (T)v.y;
     }
+}
+
+// z-plane columns through each XCD's t-slab when nt divides over the XCDs
+// (QPHIX_OPROD_ORDER=0: plain site order).
+static SiteOrder oprod_order() {
+  const Geometry &g = ctx().g;
+  SiteOrder o{0, g.ny, 1, 0};
+  const char *e = std::getenv("QPHIX_OPROD_ORDER");
+  if ((e && std::atoi(e) == 0) || g.nt % N_XCD) return o;
+  o.tslab = g.nt / N_XCD;
+  o.on = 1;
+  return o;
 }
 
 // Device part: Y_k and the outer products, into o1/o3 (SoA, both parities).
@@ -152,8 +193,8 @@ static bool oprod_device(const LinksT<T> &L, int nterms, const double *coef,
     HIP_CHECK(hipEventCreate(&e1));
   }
   if (nterms == 0) {
-    HIP_CHECK(hipMemsetAsync(o1, 0, (size_t)72 * vh * sizeof(double2), ctx().stream));
-    HIP_CHECK(hipMemsetAsync(o3, 0, (size_t)72 * vh * sizeof(double2), ctx().stream));
+    HIP_CHECK(hipMemsetAsync(o1, 0, oprod_len(vh) * sizeof(double2), ctx().stream));
+    HIP_CHECK(hipMemsetAsync(o3, 0, oprod_len(vh) * sizeof(double2), ctx().stream));
   }
   for (int k0 = 0; k0 < nterms; k0 += OPROD_GROUP) {
     const int nt = std::min(OPROD_GROUP, nterms - k0);
@@ -171,6 +212,7 @@ static bool oprod_device(const LinksT<T> &L, int nterms, const double *coef,
     a.o1 = o1;
     a.o3 = o3;
     a.g = Geo{gg.nxh, gg.ny, gg.nz, gg.nt, gg.vh, gg.nxy, gg.nxyz};
+    a.order = oprod_order();
     if (kernel_ms) HIP_CHECK(hipEventRecord(e0, ctx().stream));
     hipLaunchKernelGGL(k_fermion_oprod<T>, dim3((unsigned)((4 * (size_t)vh + 255) / 256), 2), dim3(256), 0,
                        ctx().stream, a);
@@ -208,12 +250,12 @@ static void fermion_force(QPHIX_info_t *info, typename Handles<T>::L *L, int nte
   if (info) info->status = QPHIX_FAIL;
   if (!oprod_check<T>(L, nterms, coef, X)) return;
   const int vh = ctx().g.vh;
-  double2 *o = (double2 *)dev_alloc((size_t)2 * 72 * vh * sizeof(double2));
+  double2 *o = (double2 *)dev_alloc(2 * oprod_len(vh) * sizeof(double2));
   if (!o) {
     if (info) info->status = QPHIX_MEM_ERROR;
     return;
   }
-  double2 *o1 = o, *o3 = o + (size_t)72 * vh;
+  double2 *o1 = o, *o3 = o + oprod_len(vh);
   if (!oprod_device<T>(*L, nterms, coef, X, o1, o3, nullptr)) {
     dev_free(o);
     if (info) info->status = QPHIX_MEM_ERROR;
@@ -253,14 +295,14 @@ static double fermion_force_time(typename Handles<T>::L *L, int nterms, const do
                                  typename Handles<T>::V *const *X, int reps, double *oprod_ms) {
   if (!oprod_check<T>(L, nterms, coef, X)) return -1.0;
   const int vh = ctx().g.vh;
-  double2 *o = (double2 *)dev_alloc((size_t)2 * 72 * vh * sizeof(double2));
+  double2 *o = (double2 *)dev_alloc(2 * oprod_len(vh) * sizeof(double2));
   if (!o) return -1.0;
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
-  oprod_device<T>(*L, nterms, coef, X, o, o + (size_t)72 * vh, nullptr);  // warm-up
+  oprod_device<T>(*L, nterms, coef, X, o, o + oprod_len(vh), nullptr);  // warm-up
   HIP_CHECK(hipEventRecord(e0, ctx().stream));
-  for (int r = 0; r < reps; ++r) oprod_device<T>(*L, nterms, coef, X, o, o + (size_t)72 * vh, nullptr);
+  for (int r = 0; r < reps; ++r) oprod_device<T>(*L, nterms, coef, X, o, o + oprod_len(vh), nullptr);
   HIP_CHECK(hipEventRecord(e1, ctx().stream));
   HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -269,7 +311,7 @@ static double fermion_force_time(typename Handles<T>::L *L, int nterms, const do
     double acc = 0;
     for (int r = 0; r < reps; ++r) {
       float k = 0.f;
-      oprod_device<T>(*L, nterms, coef, X, o, o + (size_t)72 * vh, &k);
+      oprod_device<T>(*L, nterms, coef, X, o, o + oprod_len(vh), &k);
       acc += k;
     }
     *oprod_ms = acc / reps;
