@@ -65,10 +65,6 @@ __device__ __forceinline__ void fwd_hops(const Geo &g, int i, int cb, int mu, in
 // directions of a site sit in neighbouring lanes, so psi(y) is loaded once per
 // site from L1 and the four outputs of each element are one contiguous 64 B
 // store.
-#ifndef QPHIX_OPROD_TILE
-#define QPHIX_OPROD_TILE 1
-#endif
-#if QPHIX_OPROD_TILE
 // Layout [cb][tile of 16 sites][element][site % 16][mu]: the 64 work-items of
 // 16 consecutive sites write one contiguous 9 KB record per output.
 constexpr int OPROD_TS = 16;
@@ -78,12 +74,6 @@ __device__ __forceinline__ size_t oprod_index(int cb, int e, int i, int mu, int 
 }
 // double2 elements of one output (both parities)
 static inline size_t oprod_len(int vh) { return 2 * oprod_tiles(vh) * 9 * 4 * OPROD_TS; }
-#else
-__device__ __forceinline__ size_t oprod_index(int cb, int e, int i, int mu, int vh) {
-  return ((size_t)(cb * 9 + e) * vh + i) * 4 + mu;
-}
-static inline size_t oprod_len(int vh) { return (size_t)72 * vh; }
-#endif
 
 // Outputs are written once and read much later (the next group or the export),
 // so they are stored non-temporally.
