@@ -346,20 +346,58 @@ static int g_sloppy_link_bits = 32;
 int sloppy_link_bits() { return g_sloppy_link_bits; }
 
 // ------------------------------------------------------------------ V
+// Raw MILC order [site][colour][re, im] <-> the tile-major spinor storage
+// (Spin<T>), one work-item per site of one parity.
+template <typename T> __global__ void k_spin_from_raw(T *dst, const T *raw, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  cplx<T> v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v[c] = cplx<T>{raw[6 * (size_t)i + 2 * c], raw[6 * (size_t)i + 2 * c + 1]};
+  Spin<T>::store(dst, i, v);
+}
+
+template <typename T> __global__ void k_spin_to_raw(T *raw, const T *src, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  cplx<T> v[3];
+  Spin<T>::load(src, i, v);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    raw[6 * (size_t)i + 2 * c] = v[c].re;
+    raw[6 * (size_t)i + 2 * c + 1] = v[c].im;
+  }
+}
+
+// qphix_d3.cpp:79-174 (create_V): the raw half of each requested parity is
+// copied to the device and reordered there.
 template <typename T>
 static typename Handles<T>::V *create_V(const T *src, int evenodd) {
   QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called:create_V", nullptr);
   const int vh = ctx().g.vh;
+  const size_t nv = 6 * (size_t)vh;
+  T *stage = nullptr;
+  if (src) {
+    stage = (T *)dev_alloc(sizeof(T) * nv);
+    QPHIX_REQUIRE(stage, "create_V: out of device memory", nullptr);
+  }
   auto *V = new typename Handles<T>::V;
   V->parity = evenodd;
   for (int cb = 0; cb < 2; ++cb) {
     const int bit = cb ? QPHIX_ODD : QPHIX_EVEN;
     if (!(evenodd & bit)) continue;
-    V->half[cb] = (T *)dev_alloc(sizeof(T) * 6 * (size_t)vh);
-    if (src) h2d(V->half[cb], src + (cb ? 6 * (size_t)vh : 0), sizeof(T) * 6 * (size_t)vh);
-    else HIP_CHECK(hipMemsetAsync(V->half[cb], 0, sizeof(T) * 6 * (size_t)vh, ctx().stream));
+    V->half[cb] = (T *)dev_alloc(sizeof(T) * nv);
+    if (src) {
+      h2d(stage, src + (cb ? nv : 0), sizeof(T) * nv);
+      hipLaunchKernelGGL(k_spin_from_raw<T>, dim3((vh + 255) / 256), dim3(256), 0, ctx().stream,
+                         V->half[cb], stage, vh);
+      HIP_CHECK(hipGetLastError());
+    } else {
+      HIP_CHECK(hipMemsetAsync(V->half[cb], 0, sizeof(T) * nv, ctx().stream));
+    }
   }
   sync();
+  dev_free(stage);
   return V;
 }
 
@@ -369,9 +407,18 @@ template <typename T> static void extract_V(T *dest, const typename Handles<T>::
   QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called:extract_V");
   QPHIX_REQUIRE(V && dest, "extract_V: null argument");
   const int vh = ctx().g.vh;
+  const size_t nv = 6 * (size_t)vh;
+  T *stage = (T *)dev_alloc(sizeof(T) * nv);
+  QPHIX_REQUIRE(stage, "extract_V: out of device memory");
   for (int cb = 0; cb < 2; ++cb)
-    if (V->half[cb]) d2h(dest + (cb ? 6 * (size_t)vh : 0), V->half[cb], sizeof(T) * 6 * (size_t)vh);
+    if (V->half[cb]) {
+      hipLaunchKernelGGL(k_spin_to_raw<T>, dim3((vh + 255) / 256), dim3(256), 0, ctx().stream, stage,
+                         V->half[cb], vh);
+      HIP_CHECK(hipGetLastError());
+      d2h(dest + (cb ? nv : 0), stage, sizeof(T) * nv);
+    }
   sync();
+  dev_free(stage);
 }
 
 template <typename T> static void destroy_V(typename Handles<T>::V *V) {

@@ -157,10 +157,12 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     k_mx_init(float *rs, const double *r, P *p, float *xs, int vh) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
+  cplx<double> rd[3];
+  Spin<double>::load(r, i, rd);
   cplx<float> v[3];
   for (int c = 0; c < 3; ++c) {
-    v[c].re = (float)r[6 * (size_t)i + 2 * c];
-    v[c].im = (float)r[6 * (size_t)i + 2 * c + 1];
+    v[c].re = (float)rd[c].re;
+    v[c].im = (float)rd[c].im;
   }
   Spin<float>::store(rs, i, v);
   Spin<P>::store(p, i, v);
@@ -176,10 +178,12 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   double s[1] = {0.0};
   if (i < vh) {
     cplx<float> v[3], pv[3];
+    cplx<double> rd[3];
     Spin<P>::load(p, i, pv);
+    Spin<double>::load(r, i, rd);
     for (int c = 0; c < 3; ++c) {
-      v[c].re = (float)r[6 * (size_t)i + 2 * c];
-      v[c].im = (float)r[6 * (size_t)i + 2 * c + 1];
+      v[c].re = (float)rd[c].re;
+      v[c].im = (float)rd[c].im;
       s[0] += (double)v[c].re * pv[c].re + (double)v[c].im * pv[c].im;
     }
     Spin<float>::store(rs, i, v);

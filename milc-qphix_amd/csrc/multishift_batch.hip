@@ -207,13 +207,16 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   const size_t stride = (size_t)vh * NR;
-  for (int e = 0; e < 6; ++e) {
-    const T v = (T)b[6 * (size_t)i + e];
-    rs[6 * ((size_t)i * NR + r) + e] = v;
-    for (int j = 0; j < nshift; ++j) {
-      p[6 * (stride * j + (size_t)i * NR + r) + e] = v;
-      xs[6 * (stride * j + (size_t)i * NR + r) + e] = (T)0;
-    }
+  const int t = i * NR + r;
+  cplx<double> bd[3];
+  Spin<double>::load(b, i, bd);
+  cplx<T> v[3];
+  const cplx<T> z[3] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int c = 0; c < 3; ++c) v[c] = cplx<T>{(T)bd[c].re, (T)bd[c].im};
+  Spin<T>::store(rs, t, v);
+  for (int j = 0; j < nshift; ++j) {
+    Spin<T>::store(p + 6 * stride * j, t, v);
+    Spin<T>::store(xs + 6 * stride * j, t, z);
   }
 }
 
@@ -224,13 +227,19 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   const size_t stride = (size_t)vh * NR;
+  const int t = i * NR + r;
+  const cplx<T> z[3] = {{0, 0}, {0, 0}, {0, 0}};
   for (int j = 0; j < nshift; ++j) {
-    T *s = xs + 6 * (stride * j + (size_t)i * NR + r);
-    double *d = x64[j] + 6 * (size_t)i;
-    for (int e = 0; e < 6; ++e) {
-      d[e] += (double)s[e];
-      s[e] = (T)0;
+    cplx<T> sv[3];
+    cplx<double> dv[3];
+    Spin<T>::load(xs + 6 * stride * j, t, sv);
+    Spin<double>::load(x64[j], i, dv);
+    for (int c = 0; c < 3; ++c) {
+      dv[c].re += (double)sv[c].re;
+      dv[c].im += (double)sv[c].im;
     }
+    Spin<double>::store(x64[j], i, dv);
+    Spin<T>::store(xs + 6 * stride * j, t, z);
   }
 }
 
@@ -239,7 +248,11 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     k_msx_set_r(float *rs, const double *r64, int vh, int NR, int r) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
-  for (int e = 0; e < 6; ++e) rs[6 * ((size_t)i * NR + r) + e] = (float)r64[6 * (size_t)i + e];
+  cplx<double> rd[3];
+  Spin<double>::load(r64, i, rd);
+  cplx<float> v[3];
+  for (int c = 0; c < 3; ++c) v[c] = cplx<float>{(float)rd[c].re, (float)rd[c].im};
+  Spin<float>::store(rs, i * NR + r, v);
 }
 
 // Resume a paused source with the FP64 true residual's norm; delta > 0 also
@@ -427,7 +440,9 @@ __global__ void k_mrhs_pack(float *dst, const float *const *src, int vh, int n, 
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= vh * NR) return;
   const int i = t / NR, r = t % NR;
-  for (int e = 0; e < 6; ++e) dst[6 * (size_t)t + e] = r < n ? src[r][6 * (size_t)i + e] : 0.f;
+  cplx<float> v[3] = {{0, 0}, {0, 0}, {0, 0}};
+  if (r < n) Spin<float>::load(src[r], i, v);
+  Spin<float>::store(dst, t, v);
 }
 
 __global__ void k_mrhs_unpack(float *const *dst, const float *src, int vh, int n, int NR) {
@@ -435,7 +450,9 @@ __global__ void k_mrhs_unpack(float *const *dst, const float *src, int vh, int n
   if (t >= vh * NR) return;
   const int i = t / NR, r = t % NR;
   if (r >= n) return;
-  for (int e = 0; e < 6; ++e) dst[r][6 * (size_t)i + e] = src[6 * (size_t)t + e];
+  cplx<float> v[3];
+  Spin<float>::load(src, t, v);
+  Spin<float>::store(dst[r], i, v);
 }
 
 template <int NR>
