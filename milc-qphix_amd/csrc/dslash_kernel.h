@@ -124,17 +124,16 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) QPHIX_DSLASH_OCC k_dslash(const 
     site_neighbours(a.g, i, a.cb, n);
     unsigned sg = 0;
     if constexpr (NL == 12) sg = a.lsign[i];
-    // tile-major links (link_chunk): chunk k of direction mu at
-    // tile base + (mu * NCH + k) * LINK_TILE + i % LINK_TILE
-    const size_t tile = (size_t)(i / LINK_TILE);
-    const typename vec16<TL>::type *fb = a.fat + tile * (4 * NCH_F * LINK_TILE);
-    const typename vec16<TL>::type *lb = a.lng + tile * (4 * NCH_L * LINK_TILE);
+    // grouped tile-major links (link_chunk): chunk k of direction mu at
+    // link_base + (mu * NCH + k) * LINK_CSTRIDE + i % LINK_TILE
+    const typename vec16<TL>::type *fb = a.fat + link_base(i, NCH_F);
+    const typename vec16<TL>::type *lb = a.lng + link_base(i, NCH_L);
     const int ii = i % LINK_TILE;
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) {
-      apply_pair<TL, 18, SI, NTL>(fb + (size_t)mu * NCH_F * LINK_TILE, LINK_TILE, ii, a.in,
+      apply_pair<TL, 18, SI, NTL>(fb + (size_t)mu * NCH_F * LINK_CSTRIDE, LINK_CSTRIDE, ii, a.in,
                                   n.f1[mu], n.b1[mu], 0, 0, acc);
-      apply_pair<TL, NL, SI, NTL>(lb + (size_t)mu * NCH_L * LINK_TILE, LINK_TILE, ii, a.in,
+      apply_pair<TL, NL, SI, NTL>(lb + (size_t)mu * NCH_L * LINK_CSTRIDE, LINK_CSTRIDE, ii, a.in,
                                   n.f3[mu], n.b3[mu], (sg >> (2 * mu + 1)) & 1u,
                                   (sg >> (2 * mu)) & 1u, acc);
     }

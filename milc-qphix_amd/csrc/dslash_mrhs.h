@@ -96,13 +96,14 @@ __device__ __forceinline__ void apply_links_k(cplx<T> *u, const T *in, int nf, i
   }
 }
 
-// The same with the link pair loaded from the tile-major link array.
+// The same with the link pair loaded from the link array (link_chunk layout;
+// base = link_base + direction offset, chunks LINK_CSTRIDE apart).
 template <typename T, int NC, int NR, int K>
 __device__ __forceinline__ void apply_pair_k(const typename vec16<T>::type *base, int ii, const T *in,
                                              int nf, int nb, int r0, unsigned sg_f, unsigned sg_b,
                                              cplx<T> (*acc)[3]) {
   cplx<T> u[NC];
-  PairLoad<T, NC, true>::load(base, LINK_TILE, ii, u);
+  PairLoad<T, NC, true>::load(base, LINK_CSTRIDE, ii, u);
   apply_links_k<T, NC, NR, K>(u, in, nf, nb, r0, sg_f, sg_b, acc);
 }
 
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs(const MrhsArgsT<T>
     site_neighbours(a.g, i, a.cb, n);
     unsigned sg = 0;
     if constexpr (NL == 12) sg = a.lsign[i];
-    const size_t tile = (size_t)(i / LINK_TILE);
+    const size_t fb = link_base(i, NCH_F), lb = link_base(i, NCH_L);
     const int ii = i % LINK_TILE;
     // With K > 1 an unrolled direction loop lets the scheduler hoist all 16 x K
     // gathers and overflow the register file; one direction at a time keeps
@@ -167,9 +168,9 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs(const MrhsArgsT<T>
     constexpr int MU_UNROLL = K == 1 ? 4 : 1;
 #pragma unroll MU_UNROLL
     for (int mu = 0; mu < 4; ++mu) {
-      apply_pair_k<T, 18, NR, K>(a.fat + (tile * 4 * NCH_F + mu * NCH_F) * LINK_TILE, ii, a.in, n.f1[mu],
+      apply_pair_k<T, 18, NR, K>(a.fat + fb + (size_t)mu * NCH_F * LINK_CSTRIDE, ii, a.in, n.f1[mu],
                                  n.b1[mu], r0, 0, 0, acc);
-      apply_pair_k<T, NL, NR, K>(a.lng + (tile * 4 * NCH_L + mu * NCH_L) * LINK_TILE, ii, a.in, n.f3[mu],
+      apply_pair_k<T, NL, NR, K>(a.lng + lb + (size_t)mu * NCH_L * LINK_CSTRIDE, ii, a.in, n.f3[mu],
                                  n.b3[mu], r0, (sg >> (2 * mu + 1)) & 1u, (sg >> (2 * mu)) & 1u, acc);
     }
 #pragma unroll
