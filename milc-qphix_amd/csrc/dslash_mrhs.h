@@ -3,9 +3,10 @@
 // Links are read from HBM once per site and applied to NR spinors, so link
 // traffic (96% of a single-RHS Dslash's bytes) is shared: bytes per site become
 // links + NR x 2 x spinor instead of NR x (links + 2 x spinor).  Spinors of
-// the NR right-hand sides are interleaved per site ([site][rhs][colour][re,im],
-// FP32 or FP64), so one neighbour gather fetches a site's sources
-// contiguously.  Each RHS carries its own gate (its solver state's `active`);
+// the NR right-hand sides are interleaved per site (index site * NR + rhs,
+// FP32 or FP64, stored by Spin<T> tile-major over 64 such indices), so one
+// neighbour gather fetches a site's sources from one contiguous stretch per
+// colour.  Each RHS carries its own gate (its solver state's `active`);
 // inactive RHS are not stored.
 //
 // Thread mapping: a work-item handles K of the NR sources of one site, so
