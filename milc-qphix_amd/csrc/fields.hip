@@ -387,6 +387,13 @@ static typename Handles<T>::V *create_V(const T *src, int evenodd) {
     const int bit = cb ? QPHIX_ODD : QPHIX_EVEN;
     if (!(evenodd & bit)) continue;
     V->half[cb] = (T *)dev_alloc(sizeof(T) * nv);
+    if (!V->half[cb]) {  // dev_alloc has recorded the error
+      dev_free(V->half[0]);
+      dev_free(V->half[1]);
+      dev_free(stage);
+      delete V;
+      return nullptr;
+    }
     if (src) {
       h2d(stage, src + (cb ? nv : 0), sizeof(T) * nv);
       hipLaunchKernelGGL(k_spin_from_raw<T>, dim3((vh + 255) / 256), dim3(256), 0, ctx().stream,
