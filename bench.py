@@ -21,6 +21,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "milc-qphix_amd"))
+# OpenMP placement of the CPU baseline's threads (read when the OpenMP runtime
+# starts, so set before anything loads it); an explicit setting wins.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 import numpy as np  # noqa: E402
 
@@ -60,21 +64,36 @@ def make_fields(lat, F):
     return raw, src
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(dims, raw, src, budget_s):
-    """Oracle Dslash (FP64) timed on the host: bounded number of applications."""
+    """Oracle Dslash (FP64) timed on the host: two warm-ups, then a bounded
+    number (5 to 50) of individually timed applications; returns the median."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     orc = O.Oracle(dims, *raw)
     vh = int(np.prod(dims)) // 2
     half = np.ascontiguousarray(src[6 * vh:])
     t = time.perf_counter()
-    orc.dslash(half, 0)  # warm-up
-    first = time.perf_counter() - t
-    reps = max(2, min(50, int(budget_s / max(first, 1e-3))))
-    t = time.perf_counter()
+    orc.dslash(half, 0)  # warm-ups
+    orc.dslash(half, 0)
+    first = (time.perf_counter() - t) / 2
+    reps = max(5, min(50, int(budget_s / max(first, 1e-3))))
+    ts = []
     for _ in range(reps):
+        t = time.perf_counter()
         orc.dslash(half, 0)
-    dt = (time.perf_counter() - t) / reps
+        ts.append(time.perf_counter() - t)
+    dt = float(np.median(ts))
     threads = O.lib().oracle_num_threads()
     orc.close()
     return dt, reps, threads
@@ -426,8 +445,10 @@ def main():
         dt, reps, threads = cpu_baseline(DIMS, raw, src, args.cpu_seconds)
         cpu = {"value": round(rb * lat.vh / dt / 1e9, 2), "unit": "GB/s", "cores": threads,
                "kind": "port",
-               "sample": f"{reps} FP64 single-parity Dslash applications on the 32^4 weak-field "
-                         f"lattice, oracle OpenMP site loop, {dt * 1e3:.1f} ms each"}
+               "sample": f"median of {reps} FP64 single-parity Dslash applications (after 2 "
+                         f"warm-ups) on the 32^4 weak-field lattice, oracle OpenMP site loop, "
+                         f"{dt * 1e3:.1f} ms each; reported baseline only",
+               "cpu_model": cpu_model()}
         log(f"[bench] CPU oracle Dslash {dt * 1e3:.1f} ms ({threads} threads)")
 
     out = {
