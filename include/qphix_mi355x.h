@@ -59,11 +59,11 @@ void QPHIX_MI355X_set_graphs(int on);
  * per source); on == 0: every work-item loads its own links.  Both give
  * identical results. */
 void QPHIX_MI355X_set_mrhs_lds(int on);
-/* on != 0 (default): the batched multi-shift solver updates the shifted
- * systems' x_j and p_j two steps at a time (each read and written once per
- * two steps; the base system every step); on == 0: every step.  Both give
- * identical results. */
-void QPHIX_MI355X_set_msx_fuse(int on);
+/* The batched multi-shift solver updates the shifted systems' x_j and p_j
+ * `steps` steps at a time (each read and written once per group of steps; the
+ * base system every step): 4 (the default and the largest), 3, 2, or 1 (every
+ * step; 0 and negative values mean 1).  All give identical results. */
+void QPHIX_MI355X_set_msx_fuse(int steps);
 /* Links of the mixed solvers' sloppy operator (QPHIX_D3_asqtad_invert_mixed):
  * 32 (default) = FP32, 16 = 16-bit fixed point (per link a float scale, the
  * largest |element|, and int16 elements; ~1.5e-5 relative per element), FP32

@@ -111,11 +111,11 @@ __device__ __forceinline__ void msx_one(cplx<T> *xv, cplx<T> *pv, const cplx<T> 
 // x_j += beta_j p_j ; p_j = zeta_j r + alpha_j p_j  for every shift j < num_now
 // not yet retired, one work-item per (site, source); layout
 // [shift][site][source][6].  Only the base system's p feeds the operator, so
-// the shifted systems go two steps at a time: on a deferred step (st.defer)
-// only the base system is updated and r is saved in rprev; on the next step
-// (st.pend) each shifted system does both steps in registers, reading and
-// writing x_j and p_j once for two steps (same arithmetic and order as two
-// single steps).
+// the shifted systems go up to st.fuse steps at a time: on a deferred step
+// (st.defer) only the base system is updated and r is saved in rprev slot
+// st.pend; on the next step that is not deferred each shifted system does
+// every pending step and then this one in registers, reading and writing x_j
+// and p_j once for all of them (same arithmetic and order as single steps).
 #ifndef MSX_GROUP
 #define MSX_GROUP 2
 #endif
@@ -140,39 +140,45 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     Spin<T>::store(p + 6 * stride * j, t, pv);
   };
   if (s.defer) {
-    Spin<T>::store(rprev, t, rv);
+    Spin<T>::store(rprev + 6 * stride * s.pend, t, rv);
     step1(jl);
     return;
   }
   // Shifts MSX_GROUP at a time: the group's x_j, p_j loads are issued before
   // any store, so a work-item's shifts are not one dependent load-store chain.
-  cplx<T> r1[3];
-  const bool pend = s.pend != 0;
-  if (pend) Spin<T>::load(rprev, t, r1);
-  // num_now only shrinks, so a pending step covers every shift of this one
-  const int lim = min(pend ? s.pend_nn : nn, nshift);
+  const int np = s.pend;
+  cplx<T> rq[MSX_MAXDEFER - 1][3];
+#pragma unroll
+  for (int q = 0; q < MSX_MAXDEFER - 1; ++q)
+    if (q < np) Spin<T>::load(rprev + 6 * stride * q, t, rq[q]);
+  // num_now only shrinks, so the first pending step covers every shift of the later ones
+  const int lim = min(np ? s.pend_nn[0] : nn, nshift);
   for (int j0 = 0; j0 < lim; j0 += MSX_GROUP) {
     bool on[MSX_GROUP];
     cplx<T> xv[MSX_GROUP][3], pv[MSX_GROUP][3];
 #pragma unroll
-    for (int q = 0; q < MSX_GROUP; ++q) {
-      const int j = j0 + q;
-      on[q] = j < lim && s.sdone[j] != 1;  // retired earlier: no update
-      if (on[q]) {
-        Spin<T>::load(xs + 6 * stride * j, t, xv[q]);
-        Spin<T>::load(p + 6 * stride * j, t, pv[q]);
+    for (int g = 0; g < MSX_GROUP; ++g) {
+      const int j = j0 + g;
+      on[g] = j < lim && s.sdone[j] != 1;  // retired earlier: no update
+      if (on[g]) {
+        Spin<T>::load(xs + 6 * stride * j, t, xv[g]);
+        Spin<T>::load(p + 6 * stride * j, t, pv[g]);
       }
     }
 #pragma unroll
-    for (int q = 0; q < MSX_GROUP; ++q) {
-      if (!on[q]) continue;
-      const int j = j0 + q;
-      if (pend && j != jl)  // the pending step, then this one
-        msx_one<T>(xv[q], pv[q], r1, (T)s.pbeta[j], (T)s.palpha[j], (T)s.pzeta[j]);
-      if (j < nn || j == jl)
-        msx_one<T>(xv[q], pv[q], rv, (T)s.beta_i[j], (T)s.alpha[j], (T)s.zeta_ip1[j]);
-      Spin<T>::store(xs + 6 * stride * j, t, xv[q]);
-      Spin<T>::store(p + 6 * stride * j, t, pv[q]);
+    for (int g = 0; g < MSX_GROUP; ++g) {
+      if (!on[g]) continue;
+      const int j = j0 + g;
+      if (j != jl) {  // the pending steps in order (the base system took them already)
+#pragma unroll
+        for (int q = 0; q < MSX_MAXDEFER - 1; ++q)
+          if (q < np && j < s.pend_nn[q])
+            msx_one<T>(xv[g], pv[g], rq[q], (T)s.pbeta[q][j], (T)s.palpha[q][j], (T)s.pzeta[q][j]);
+      }
+      if (j < nn || j == jl)  // then this one
+        msx_one<T>(xv[g], pv[g], rv, (T)s.beta_i[j], (T)s.alpha[j], (T)s.zeta_ip1[j]);
+      Spin<T>::store(xs + 6 * stride * j, t, xv[g]);
+      Spin<T>::store(p + 6 * stride * j, t, pv[g]);
     }
   }
 }
@@ -185,13 +191,13 @@ __global__ void k_msx_commit(SolverState *st, int nr) {
   SolverState &s = st[r];
   for (int j = 0; j < s.num_now; ++j)
     if (s.sdone[j] == 2) s.sdone[j] = 1;
-  if (s.defer) {  // the shifted updates of this step wait for the next one
-    s.pend = 1;
-    s.pend_nn = s.num_now;
+  if (s.defer) {  // the shifted updates of this step wait for a later one
+    const int q = s.pend++;
+    s.pend_nn[q] = s.num_now;
     for (int j = 0; j < s.num_now; ++j) {
-      s.pbeta[j] = s.beta_i[j];
-      s.palpha[j] = s.alpha[j];
-      s.pzeta[j] = s.zeta_ip1[j];
+      s.pbeta[q][j] = s.beta_i[j];
+      s.palpha[q][j] = s.alpha[j];
+      s.pzeta[q][j] = s.zeta_ip1[j];
     }
   } else {
     s.pend = 0;
@@ -301,7 +307,7 @@ inline int mrhs_nlog(int vh, int NR, int K) {
 static int g_mrhs_lds = -1;
 // Two-step updates of the shifted systems in the batched multi-shift solver
 // (k_msx_update_pm; identical results): on unless switched off.
-static int g_msx_fuse = 1;
+static int g_msx_fuse = MSX_MAXDEFER;  // steps per shifted update
 static bool mrhs_lds_on() {
   if (g_mrhs_lds < 0) {
     const char *e = std::getenv("QPHIX_MRHS_LDS");
@@ -527,7 +533,7 @@ using namespace qphix;
 extern "C" {
 
 void QPHIX_MI355X_set_mrhs_lds(int on) { g_mrhs_lds = on ? 1 : 0; }
-void QPHIX_MI355X_set_msx_fuse(int on) { g_msx_fuse = on ? 1 : 0; }
+void QPHIX_MI355X_set_msx_fuse(int steps) { g_msx_fuse = std::max(1, std::min(steps, MSX_MAXDEFER)); }
 
 void QPHIX_F3_asqtad_dslash_multi(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVector *out[],
                                   QPHIX_F3_ColorVector *in[], int nvec, QPHIX_evenodd_t parity) {

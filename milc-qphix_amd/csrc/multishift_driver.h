@@ -55,13 +55,15 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
   const size_t nv = 6 * (size_t)vh, nvr = nv * NR;
   const LinksT<float> &Lf = *L->sloppy;
   const int other = 1 - parity;
-  // FP32 workspace: p, xs [nshift][vh][NR][6]; rs, Ap, tmp, rprev [vh][NR][6]; r64, tmp64; states; pointers
-  char *w = (char *)ws2_get((2 * (size_t)nshift + 4) * nvr * sizeof(float) + 2 * nv * sizeof(double) +
+  // FP32 workspace: p, xs [nshift][vh][NR][6]; rs, Ap, tmp [vh][NR][6]; rprev
+  // [MSX_MAXDEFER - 1][vh][NR][6]; r64, tmp64; states; pointers
+  constexpr size_t NRP = MSX_MAXDEFER - 1;
+  char *w = (char *)ws2_get((2 * (size_t)nshift + 3 + NRP) * nvr * sizeof(float) + 2 * nv * sizeof(double) +
                             (NR + 2) * sizeof(SolverState) + NR * nshift * sizeof(double *) + 4096);
   if (!w) return false;
   float *p = (float *)w, *xs = p + nshift * nvr, *rs = xs + nshift * nvr, *Ap = rs + nvr, *tmp = Ap + nvr;
   float *rprev = tmp + nvr;
-  double *r64 = (double *)(rprev + nvr), *tmp64 = r64 + nv;
+  double *r64 = (double *)(rprev + NRP * nvr), *tmp64 = r64 + nv;
   SolverState *st = (SolverState *)(((uintptr_t)(tmp64 + nv) + 255) & ~(uintptr_t)255);
   SolverState *tst = st + NR;
   double **dx = (double **)(tst + 1);
@@ -69,7 +71,7 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
   bool hi = switch_frac > 0;
   double *w64 = nullptr;
   if (hi) {
-    w64 = (double *)dev_alloc((2 * (size_t)nshift + 4) * nvr * sizeof(double));
+    w64 = (double *)dev_alloc((2 * (size_t)nshift + 3 + NRP) * nvr * sizeof(double));
     if (!w64) return false;
   }
   double *p_d = w64, *xs_d = hi ? p_d + nshift * nvr : nullptr;
@@ -138,7 +140,7 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     s.mx_delta = hi ? switch_frac : delta;
     s.max_cg = max_cg;
     s.active = 1;
-    s.fuse = (g_msx_fuse && nshift > 1) ? 1 : 0;  // one shift: nothing to defer
+    s.fuse = nshift > 1 ? g_msx_fuse : 1;  // one shift: nothing to defer
     if (hi)
       hipLaunchKernelGGL(k_msx_init<double>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs_d, p_d, xs_d,
                          b64[r], vh, NR, r, nshift);

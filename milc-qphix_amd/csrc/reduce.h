@@ -12,6 +12,7 @@
 namespace qphix {
 
 constexpr int MAXNDMASS = 150;  // ks_d_congrad_fn.h:4
+constexpr int MSX_MAXDEFER = 4;  // batched multi-shift: most steps per shifted update
 
 struct SolverState {
   double rsq, oldrsq, pkp, relrsq, src_norm;
@@ -32,14 +33,17 @@ struct SolverState {
   // update is its last), 1 retired; sdone_at = the step it converged at
   int sdone[MAXNDMASS];
   int sdone_at[MAXNDMASS];
-  // batched multi-shift, shifted systems updated two steps at a time: `defer`
-  // (set by this step's finalize) = the shifted updates of this step are left
-  // to the next one; `pend` (set by the commit) = such a step is pending, with
-  // its num_now and coefficients in pend_nn, pbeta, palpha, pzeta; `fuse` = 1
-  // allows deferring.  A step that requests a pause, or retires a shift, is
-  // never deferred, so a paused or finished source has nothing pending.
-  int defer, pend, pend_nn, fuse;
-  double pbeta[MAXNDMASS], palpha[MAXNDMASS], pzeta[MAXNDMASS];
+  // batched multi-shift, shifted systems updated up to `fuse` steps at a time:
+  // `defer` (set by this step's finalize) = the shifted updates of this step
+  // are left to a later one; `pend` (kept by the commit) = how many steps are
+  // pending, step q with its num_now and coefficients in pend_nn[q], pbeta[q],
+  // palpha[q], pzeta[q]; `fuse` = 1 updates every step.  A step that requests
+  // a pause, or retires a shift, is never deferred, so a paused or finished
+  // source has nothing pending.
+  int defer, pend, fuse, pad3_;
+  int pend_nn[MSX_MAXDEFER - 1];
+  double pbeta[MSX_MAXDEFER - 1][MAXNDMASS], palpha[MSX_MAXDEFER - 1][MAXNDMASS];
+  double pzeta[MSX_MAXDEFER - 1][MAXNDMASS];
 };
 
 enum ScalarOp {
@@ -200,7 +204,7 @@ __device__ inline void apply_scalar_op(SolverState *s, int op, const double *v) 
         retire = true;
       }
     ms_alpha_recurrence<double>(s);
-    s->defer = (s->fuse && !s->pend && !s->ru && !retire) ? 1 : 0;
+    s->defer = (s->pend + 1 < s->fuse && !s->ru && !retire) ? 1 : 0;
     break;
   }
   case SOP_MX_RP:

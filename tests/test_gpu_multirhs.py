@@ -167,12 +167,13 @@ def test_lds_staged_links_match_per_lane_loads():
         o.free()
 
 
-@pytest.mark.parametrize("block", [8, 4])
-def test_two_step_shifted_updates_match_single_steps(block):
-    """The batched solver's shifted systems updated two steps at a time (one
-    read and write of x_j, p_j per two steps) give the same solutions, bit for
-    bit, as one step at a time: through shift retirements, reliable-update
-    pauses, the FP64-to-FP32 switch and the per-shift refinement."""
+@pytest.mark.parametrize("block,steps", [(8, 4), (4, 4), (8, 3), (8, 2)])
+def test_deferred_shifted_updates_match_single_steps(block, steps):
+    """The batched solver's shifted systems updated `steps` steps at a time
+    (one read and write of x_j, p_j per group of steps) give the same solutions,
+    bit for bit, as one step at a time: through shift retirements,
+    reliable-update pauses, the FP64-to-FP32 switch and the per-shift
+    refinement."""
     dims = (8, 8, 8, 8)
     s = lattice_session(dims)
     lat = F.Lattice(dims)
@@ -184,9 +185,9 @@ def test_two_step_shifted_updates_match_single_steps(block):
     vb = [A.Vector(s, b, A.EVEN) for b in bs]
     got = {}
     try:
-        for fuse in (1, 0):
+        for fuse in (steps, 1):
             s.lib.QPHIX_MI355X_set_msx_fuse(fuse)
-            print(f"[case] two-step shifted updates fuse={fuse} block={block}", flush=True)
+            print(f"[case] shifted updates every {fuse} steps, block={block}", flush=True)
             vx = [[A.Vector(s, None, A.EVEN) for _ in masses] for _ in bs]
             ret, info, res = A.invert_multi_rhs(L, vx, vb, masses, A.EVEN, 1e-18, niter=1000,
                                                 nrestart=2, rhs_block=block)
@@ -197,10 +198,10 @@ def test_two_step_shifted_updates_match_single_steps(block):
                 for x in row:
                     x.free()
     finally:
-        s.lib.QPHIX_MI355X_set_msx_fuse(1)
-    assert got[1][0] == got[0][0]
-    assert got[1][2] == got[0][2]
-    for r1, r0 in zip(got[1][1], got[0][1]):
+        s.lib.QPHIX_MI355X_set_msx_fuse(4)  # the default
+    assert got[steps][0] == got[1][0]
+    assert got[steps][2] == got[1][2]
+    for r1, r0 in zip(got[steps][1], got[1][1]):
         for x, y in zip(r1, r0):
             np.testing.assert_array_equal(x, y)
     for o in [L] + vb:
