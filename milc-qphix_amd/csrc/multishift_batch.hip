@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
 }
 
 // One step of shift j: x += beta p ; p = zeta r + alpha p (explicit fmas: the
-// one-step and two-step paths below round identically).
+// single-step and grouped paths below round identically).
 template <typename T>
 __device__ __forceinline__ void msx_one(cplx<T> *xv, cplx<T> *pv, const cplx<T> *rv, T bj, T aj, T zj) {
 #pragma unroll
@@ -305,9 +305,9 @@ inline int mrhs_nlog(int vh, int NR, int K) {
 // link kernel (k_dslash_mrhs_lds; bit-identical results) unless switched off
 // (QPHIX_MI355X_set_mrhs_lds(0), or QPHIX_MRHS_LDS=0 before the first call).
 static int g_mrhs_lds = -1;
-// Two-step updates of the shifted systems in the batched multi-shift solver
-// (k_msx_update_pm; identical results): on unless switched off.
-static int g_msx_fuse = MSX_MAXDEFER;  // steps per shifted update
+// Steps per update of the shifted systems in the batched multi-shift solver
+// (k_msx_update_pm; identical results for every value): QPHIX_MI355X_set_msx_fuse.
+static int g_msx_fuse = MSX_MAXDEFER;
 static bool mrhs_lds_on() {
   if (g_mrhs_lds < 0) {
     const char *e = std::getenv("QPHIX_MRHS_LDS");
