@@ -43,7 +43,7 @@ namespace qphix {
 
 template <int NR>
 __global__ void __launch_bounds__(FINALIZE_BLOCK)
-    k_finalize_multi(const double *partials, int nb, SolverState *st, int op) {
+    k_finalize_multi(const double *partials, int nb, SolverState *st, int op, int phase) {
   double s[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -60,7 +60,10 @@ __global__ void __launch_bounds__(FINALIZE_BLOCK)
   }
   __syncthreads();
   const int r = threadIdx.x;
-  if (r < NR && st[r].active) apply_scalar_op(st + r, op, tot + r);
+  if (r < NR && st[r].active) {
+    st[r].phase = phase;
+    apply_scalar_op(st + r, op, tot + r);
+  }
 }
 
 // r_r += beta_low,r Ap_r ; reduce |r_r|^2 per source.  One work-item per

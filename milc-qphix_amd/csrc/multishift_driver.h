@@ -25,11 +25,11 @@ static void msx_batch(const LinksT<T> &Lk, int parity, int batch, T *p, T *xs, T
     mrhs_dslash<NR, T>(Lk, 1 - parity, tmp, plow, EPI_PLAIN, (T)0, nullptr, st, K);
     mrhs_dslash<NR, T>(Lk, parity, Ap, tmp, EPI_PKP, (T)-offset_low, plow, st, K);
     hipLaunchKernelGGL(k_finalize_multi<NR>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream,
-                       ctx().partials, nbd, st, (int)SOP_MS_PKP);
+                       ctx().partials, nbd, st, (int)SOP_MS_PKP, k);
     hipLaunchKernelGGL((k_msx_update_r<T, NR>), dim3(nbr), dim3(BLAS_BLOCK), 0, ctx().stream, rs, Ap, vh, st,
                        ctx().partials, nlog);
     hipLaunchKernelGGL(k_finalize_multi<NR>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream,
-                       ctx().partials, nbr, st, (int)SOP_MSX_RSQ2);
+                       ctx().partials, nbr, st, (int)SOP_MSX_RSQ2, k);
     hipLaunchKernelGGL((k_msx_update_pm<T, NR>), dim3((vh * NR + BLAS_BLOCK - 1) / BLAS_BLOCK), dim3(BLAS_BLOCK),
                        0, ctx().stream, xs, p, rs, rprev, vh, nshift, st);
     hipLaunchKernelGGL(k_msx_commit, dim3(1), dim3(NR), 0, ctx().stream, st, NR);

@@ -39,8 +39,12 @@ struct SolverState {
   // pending, step q with its num_now and coefficients in pend_nn[q], pbeta[q],
   // palpha[q], pzeta[q]; `fuse` = 1 updates every step.  A step that requests
   // a pause, or retires a shift, is never deferred, so a paused or finished
-  // source has nothing pending.
-  int defer, pend, fuse, pad3_;
+  // source has nothing pending.  `phase` = the step's place in its batch:
+  // every source closes its group at phase fuse - 1 (mod fuse), so the
+  // sources of a wavefront (interleaved per site) do their full updates on
+  // the same steps; a wavefront with one source updating every shift costs
+  // as much as one where all do.
+  int defer, pend, fuse, phase;
   int pend_nn[MSX_MAXDEFER - 1];
   double pbeta[MSX_MAXDEFER - 1][MAXNDMASS], palpha[MSX_MAXDEFER - 1][MAXNDMASS];
   double pzeta[MSX_MAXDEFER - 1][MAXNDMASS];
@@ -204,7 +208,7 @@ __device__ inline void apply_scalar_op(SolverState *s, int op, const double *v) 
         retire = true;
       }
     ms_alpha_recurrence<double>(s);
-    s->defer = (s->pend + 1 < s->fuse && !s->ru && !retire) ? 1 : 0;
+    s->defer = (s->pend + 1 < s->fuse && s->phase % s->fuse != s->fuse - 1 && !s->ru && !retire) ? 1 : 0;
     break;
   }
   case SOP_MX_RP:
