@@ -208,7 +208,8 @@ __device__ inline void apply_scalar_op(SolverState *s, int op, const double *v) 
         retire = true;
       }
     ms_alpha_recurrence<double>(s);
-    s->defer = (s->pend + 1 < s->fuse && s->phase % s->fuse != s->fuse - 1 && !s->ru && !retire) ? 1 : 0;
+    s->defer = (s->fuse > 1 && s->pend + 1 < s->fuse && s->phase % s->fuse != s->fuse - 1 && !s->ru &&
+                !retire) ? 1 : 0;
     break;
   }
   case SOP_MX_RP:
