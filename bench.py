@@ -75,6 +75,32 @@ def cpu_model():
     return "unknown"
 
 
+def host_cores():
+    """(physical cores of the host, CPUs this process may run on): the first
+    from the (package, core) pairs in sysfs, the second from the affinity mask."""
+    phys = set()
+    base = "/sys/devices/system/cpu"
+    try:
+        for d in os.listdir(base):
+            if not (d.startswith("cpu") and d[3:].isdigit()):
+                continue
+            try:
+                with open(f"{base}/{d}/topology/physical_package_id") as f:
+                    pkg = f.read().strip()
+                with open(f"{base}/{d}/topology/core_id") as f:
+                    core = f.read().strip()
+                phys.add((pkg, core))
+            except OSError:
+                continue
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 0
+    return (len(phys) or None), aff
+
+
 def cpu_baseline(dims, raw, src, budget_s):
     """Oracle Dslash (FP64) timed on the host: two warm-ups, then a bounded
     number (5 to 50) of individually timed applications; returns the median."""
@@ -99,7 +125,7 @@ def cpu_baseline(dims, raw, src, budget_s):
     return dt, reps, threads
 
 
-def cg_compare(A, s, L64, lat, src):
+def cg_compare(A, s, L64, lat, src, raw_links):
     """FP64 CG vs mixed-precision CG (FP32 and FP16-stored sloppy spinors, FP32
     or 16-bit fixed-point sloppy links, FP64 reliable updates), each to a 1e-10
     true relative residual (resid is the squared residual, 1e-20), m = 0.02 in
@@ -109,6 +135,7 @@ def cg_compare(A, s, L64, lat, src):
     b[6 * lat.vh:] = 0
     vb = A.Vector(s, b, A.EVEN, "D")
     out = {}
+    bnorm2 = float(b[:6 * lat.vh] @ b[:6 * lat.vh])
     x64 = None
     for name, bits, lbits in (("f64", None, None), ("mixed_f32", 32, 32), ("mixed_f16", 16, 32),
                               ("mixed_f32_l16", 32, 16), ("mixed_f16_l16", 16, 16)):
@@ -135,7 +162,34 @@ def cg_compare(A, s, L64, lat, src):
         out[name] = rec
         log(f"[bench] CG {name}: {sec:.3f}s {res.final_iter} iters rsq {res.final_rsq:.2e}")
         vx.free()
-    vb.free()
+    # Pure FP32 CG (the reference's F build: FP32 links, vectors and solver
+    # scalars; no FP64 anywhere) with the same target: it stagnates, so it runs
+    # to its iteration cap.  Reported: where it stagnates, measured with the
+    # FP64 operator, and its error against the FP64 solution.
+    raw32 = [a.astype(np.float32) for a in raw_links]
+    L32 = A.Links(s, *raw32, prec="F")
+    del raw32
+    vb32 = A.Vector(s, b.astype(np.float32), A.EVEN, "F")
+    vx32 = A.Vector(s, None, A.EVEN, "F")
+    t = time.perf_counter()
+    ret, info, res = A.invert(L32, vx32, vb32, mass, A.EVEN, resid, niter=1000, nrestart=4)
+    sec = time.perf_counter() - t
+    x32 = vx32.extract()[:6 * lat.vh].astype(np.float64)
+    vx = A.Vector(s, np.concatenate([x32, np.zeros(6 * lat.vh)]), A.EVEN, "D")
+    tmp = A.Vector(s, None, A.EVENODD, "D")
+    A.dslash(L64, tmp, vx, A.ODD)
+    A.dslash(L64, vx_out := A.Vector(s, None, A.EVEN, "D"), tmp, A.EVEN)
+    ddx = vx_out.extract()[:6 * lat.vh]
+    r = b[:6 * lat.vh] - (4 * mass * mass * x32 - ddx)
+    true_rsq = float(r @ r) / bnorm2
+    out["f32"] = {"seconds": round(sec, 4), "iters": res.final_iter, "final_rsq": res.final_rsq,
+                  "true_rsq_fp64_operator": true_rsq,
+                  "rel_err_vs_f64": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64)),
+                  "status": info.status, "stagnates": bool(true_rsq > resid)}
+    log(f"[bench] CG f32 (pure): {sec:.3f}s {res.final_iter} iters, solver rsq {res.final_rsq:.2e}, "
+        f"FP64-operator rsq {true_rsq:.2e}, rel err vs f64 {out['f32']['rel_err_vs_f64']:.1e}")
+    for o in (L32, vb32, vx32, vx, tmp, vx_out, vb):
+        o.free()
     return out
 
 
@@ -409,7 +463,10 @@ def main():
     gflops = 1146.0 * lat.vh / (ms_ev * 1e-3) / 1e9
     log(f"[bench] FP64 Dslash {ms_ev * 1e3:.1f} us  {gbs:.0f} GB/s  {gflops:.0f} GFLOP/s")
 
-    extra = {"copy_kernel_GBs": round(copy_gbs, 1), "n_cu": ncu.value}
+    extra = {"plain_copy_kernel_GBs": round(copy_gbs, 1),
+             "plain_copy_note": "one grid-stride 16-byte copy of 2 GiB (read + write); a floor for "
+                                "the card, not its bandwidth ceiling (8 TB/s peak is the roofline)",
+             "n_cu": ncu.value}
     extra["dslash_f64"] = {"us": round(ms_ev * 1e3, 2), "GBs": round(gbs, 1),
                            "GFLOPs": round(gflops, 1), "bytes_per_site": rb, "recon": L64.recon}
 
@@ -438,12 +495,16 @@ def main():
     log(f"[bench] 16-bit-link Dslash {ms16 * 1e3:.1f} us  {extra['dslash_l16']['GBs']} GB/s")
 
     if not args.no_cg:
-        extra["cg"] = cg_compare(A, s, L64, lat, src)
+        extra["cg"] = cg_compare(A, s, L64, lat, src, raw)
 
     cpu = None
     if not args.no_cpu:
         dt, reps, threads = cpu_baseline(DIMS, raw, src, args.cpu_seconds)
+        phys, aff = host_cores()
         cpu = {"value": round(rb * lat.vh / dt / 1e9, 2), "unit": "GB/s", "cores": threads,
+               "cores_note": "OpenMP threads used; the affinity mask allows "
+                             f"{aff} CPUs of a host with {phys} physical cores",
+               "affinity_cpus": aff, "host_physical_cores": phys,
                "kind": "port",
                "sample": f"median of {reps} FP64 single-parity Dslash applications (after 2 "
                          f"warm-ups) on the 32^4 weak-field lattice, oracle OpenMP site loop, "

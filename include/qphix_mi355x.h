@@ -53,6 +53,19 @@ int QPHIX_F3_gauge_links_recon(QPHIX_F3_GaugeField *G);
  * on == 0: plain launches (the same kernels in the same order, hence
  * identical results). */
 void QPHIX_MI355X_set_graphs(int on);
+/* Test hook for the out-of-memory paths: after `after` more successful device
+ * allocations, every allocation by the library fails as if device memory
+ * were exhausted (the error is recorded as for a real failure); a negative
+ * value turns this off (the default). */
+void QPHIX_MI355X_debug_fail_alloc(int after);
+/* Single-reduction CG steps, the FEWSUMS variant of the reference's CG
+ * (ks_d_congrad_fn.cpp:450-541, a compile-time option there): the p.Ap
+ * reduction also sums Re<Ap|r>, |Ap|^2 and |r|^2, the new |r|^2 follows as
+ * |r|^2 + 2a Re<Ap|r> + a^2 |Ap|^2, and x, r and p are updated in one pass.
+ * mode bit 0: QPHIX_*3_asqtad_invert (when relresid is off); bit 1: the sloppy
+ * steps of QPHIX_D3_asqtad_invert_mixed.  Default 2 (QPHIX_FEWSUMS in the
+ * environment sets the initial value). */
+void QPHIX_MI355X_set_fewsums(int mode);
 /* on != 0 (default; QPHIX_MRHS_LDS=0 in the environment sets the initial
  * value to 0): multi-RHS Dslash launches with one source per work-item stage
  * each wavefront's links through LDS (loaded once per site instead of once
@@ -93,15 +106,42 @@ void QPHIX_D3_asqtad_links_backcheck(QPHIX_D3_FermionLinksAsqtad *L, double out[
  * or FP16-stored, FP32-computed (sloppy_bits = 16) vectors; the solution, the
  * true residual b - Mx (recomputed with the FP64 Dslash whenever the sloppy
  * residual has dropped by 10x, or on apparent convergence) and all reductions
- * are FP64.  Stops when the true squared relative residual is below
- * res_arg->resid, or after inv_arg->max * inv_arg->nrestart iterations.
- * out is the initial guess.  res_arg->final_restart returns the number of
- * reliable updates; final_iter counts sloppy steps plus true-residual
- * evaluations.  Returns the iterations of the last parity solved. */
+ * are FP64.  Stops, as the reference's CG does (ks_d_congrad_fn.cpp:424-427),
+ * when the true squared relative residual is below res_arg->resid and the
+ * relative residue (ks_blas_utils_c.cpp:110-149) is below res_arg->relresid,
+ * a target <= 0 counting as met; both are evaluated on the FP64 true residual
+ * at each reliable update (with only relresid set, updates happen every 10x
+ * drop of the sloppy residual).  After inv_arg->max * inv_arg->nrestart
+ * iterations it stops with info->status = QPHIX_FAIL.  out is the initial
+ * guess.  res_arg->final_rsq/size_r and final_rel/size_relr return the last
+ * true values; final_restart the number of reliable updates; final_iter
+ * counts sloppy steps plus true-residual evaluations.  Returns the
+ * iterations of the last parity solved. */
 int QPHIX_D3_asqtad_invert_mixed(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad *asqtad,
                                  QPHIX_invert_arg_t *inv_arg, QPHIX_resid_arg_t *res_arg,
                                  double mass, QPHIX_D3_ColorVector *out, QPHIX_D3_ColorVector *in,
                                  int sloppy_bits);
+
+/* ---- gauge-field update and molecular dynamics --------------------------------
+ * U_mu(x) <- exp(eps A_mu(x)) U_mu(x) for every link of G (both parities), A
+ * the traceless anti-Hermitian momentum of F (H = -iA, MILC's 8-real layout).
+ * The reference updates the gauge field inside update_h_u_gforce_imp
+ * (gauge_force_imp.cpp:221-369, update_mg_body with epsilonU = eps, body
+ * absent); here it is a call of its own, MILC's update_u.  The exponential is
+ * exact to rounding (Cayley-Hamilton form, scaling and squaring), so U stays
+ * SU(3); a two-row copy kept for the force kernel is updated with it. */
+void QPHIX_D3_update_gauge(QPHIX_info_t *info, QPHIX_D3_GaugeField *G, QPHIX_D3_Force *F, double eps);
+void QPHIX_F3_update_gauge(QPHIX_info_t *info, QPHIX_F3_GaugeField *G, QPHIX_F3_Force *F, float eps);
+/* nsteps molecular-dynamics steps of size eps for the pure-gauge Symanzik
+ * action of coeffs (the force of QPHIX_*3_symanzik_1loop_gauge_force, the
+ * update of QPHIX_*3_update_gauge), conserving
+ * H = -1/2 sum Tr A^2 - sum_loops c Re Tr U_loop to O(eps^2).
+ * scheme 0: leapfrog; 1: Omelyan's 2nd-order minimum-norm integrator
+ * (lambda = 0.1931833275...).  G and F are updated in place. */
+void QPHIX_D3_gauge_md(QPHIX_info_t *info, QPHIX_D3_GaugeField *G, QPHIX_D3_Force *F,
+                       QPHIX_gauge_coeffs_t *coeffs, double eps, int nsteps, int scheme);
+void QPHIX_F3_gauge_md(QPHIX_info_t *info, QPHIX_F3_GaugeField *G, QPHIX_F3_Force *F,
+                       QPHIX_gauge_coeffs_t *coeffs, float eps, int nsteps, int scheme);
 
 /* Fermion-force outer products (no reference counterpart: milc-qphix has no
  * fermion force).  For the pseudofermion action
@@ -138,7 +178,9 @@ double QPHIX_F3_asqtad_fermion_force_time(QPHIX_F3_FermionLinksAsqtad *asqtad, i
 /* Multi-shift CG on nrhs sources at once (ks_multicg_offset.cpp:205-689
  * solves one source in one precision).  Solves (4 mass[j]^2 - D D) out[r*nshift+j]
  * = in[r] for every source r and shift j, each to the squared relative true
- * residual res_arg[r*nshift+j]->resid (res_arg[0]->resid where that is 0).
+ * residual res_arg[r*nshift+j]->resid (res_arg[0]->resid where that is 0;
+ * a resulting target <= 0 is rejected).  relresid is not used, as in the
+ * reference's multi-shift (ks_multicg_offset.cpp:282 reads it, no test uses it).
  * Sources advance in blocks of rhs_block (1, 2, 4 or 8; 0 = automatic) through
  * one multi-right-hand-side FP32 Dslash per half-step, with FP64 solutions and
  * FP64 reliable updates of each source's base system; afterwards every shifted

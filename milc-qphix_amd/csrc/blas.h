@@ -19,6 +19,13 @@ template <typename T> void blas_cg_axpy(const T *r, T *p, const SolverState *st,
 template <typename T>
 void blas_cg_step_c(T *x, T *p, T *r, const T *ttt, SolverState *st, int *gate, int nb_pkp);
 double *partials_b();
+// One single-reduction (FEWSUMS) CG step after an EPI_PKP_FS Dslash with
+// nb_pkp blocks of partials: x, r and p in one pass (k_cg_step_fs).
+template <typename T>
+void blas_cg_step_fs(T *x, T *p, T *r, const T *ttt, SolverState *st, int *gate, int nb_pkp);
+// Which solvers take single-reduction steps (fewsums_mode(), common.h): bit 0
+// the one-precision CG (QPHIX_*3_asqtad_invert, when relresid is off), bit 1
+// the sloppy steps of the mixed-precision CG.
 template <typename T> void blas_ms_update_r(T *r, const T *ttt, SolverState *st, const int *gate);
 template <typename T>
 void blas_ms_update_pm(T *const *x, T *const *p, const T *r, int nshift, const SolverState *st,

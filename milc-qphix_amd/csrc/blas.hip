@@ -35,19 +35,18 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     Spin<T>::load(p, i, pv);
     Spin<T>::load(r, i, rv);
     Spin<T>::load(ttt, i, tv);
-    double num = 0, den = 0;
+    double num = 0;
     for (int c = 0; c < 3; ++c) {
       xv[c].re = xv[c].re + a * pv[c].re;
       xv[c].im = xv[c].im + a * pv[c].im;
       rv[c].re = rv[c].re + a * tv[c].re;
       rv[c].im = rv[c].im + a * tv[c].im;
       num += (double)rv[c].re * rv[c].re + (double)rv[c].im * rv[c].im;
-      den += (double)xv[c].re * xv[c].re + (double)xv[c].im * xv[c].im;
     }
     Spin<T>::store(x, i, xv);
     Spin<T>::store(r, i, rv);
     s[0] = num;
-    s[1] = num == 0 ? 0.0 : (den == 0 ? 1.0 : num / den);
+    s[1] = rel_term(site_norm2_rounded(rv), site_norm2_rounded(xv));
   }
   grid_reduce<2>(s, rc);
 }
@@ -101,19 +100,18 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     Spin<T>::load(p, i, pv);
     Spin<T>::load(r, i, rv);
     Spin<T>::load(ttt, i, tv);
-    double num = 0, den = 0;
+    double num = 0;
     for (int c = 0; c < 3; ++c) {
       xv[c].re = xv[c].re + ac * pv[c].re;
       xv[c].im = xv[c].im + ac * pv[c].im;
       rv[c].re = rv[c].re + ac * tv[c].re;
       rv[c].im = rv[c].im + ac * tv[c].im;
       num += (double)rv[c].re * rv[c].re + (double)rv[c].im * rv[c].im;
-      den += (double)xv[c].re * xv[c].re + (double)xv[c].im * xv[c].im;
     }
     Spin<T>::store(x, i, xv);
     Spin<T>::store(r, i, rv);
     s[0] = num;
-    s[1] = num == 0 ? 0.0 : (den == 0 ? 1.0 : num / den);
+    s[1] = rel_term(site_norm2_rounded(rv), site_norm2_rounded(xv));
   }
   block_sum<2>(s);
   if (threadIdx.x == 0) {
@@ -158,6 +156,70 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   cplx<C> rv[3], pv[3];
   Spin<T>::load(r, i, rv);
   Spin<T>::load(p, i, pv);
+  for (int c = 0; c < 3; ++c) {
+    pv[c].re = rv[c].re + bc * pv[c].re;
+    pv[c].im = rv[c].im + bc * pv[c].im;
+  }
+  Spin<T>::store(p, i, pv);
+}
+
+// Single-reduction CG step, the FEWSUMS variant of the reference's loop
+// (ks_d_congrad_fn.cpp:450-541; MILC's algorithm, whose sums the C++ port
+// leaves uncomputed, :150-158).  The preceding EPI_PKP_FS Dslash reduced
+// pkp = p.ttt, c_tr = Re<ttt|r>, c_tt = <ttt|ttt> and the actual |r|^2 of the
+// residual before this step, and snapshotted the gate, the iteration and rsq
+// (the previous step's estimate, or the restart's true value) into fs_*.
+// Then a = -rsq / pkp; x += a p; r += a ttt; the new |r|^2 follows without a
+// second reduction, rsq' = actual + 2 a c_tr + a^2 c_tt; b = rsq' / actual;
+// and unless the inner loop ends here (restart interval or convergence on
+// rsq', ks_d_congrad_fn.cpp:566-572) p = r + b p, all in one pass.  Every
+// block derives the same scalars from the same partials; block 0 records them.
+// The relative residue needs a reduction of the updated vectors, so this step
+// is used only when relresid is off.
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_cg_step_fs(T *x, T *p, T *r, const T *ttt, int vh, SolverState *st, const double *partials, int nb,
+                 int *gate) {
+  if (st->fs_gate == 0) return;  // written by the preceding kernel: uniform
+  double v[4];
+  sum_partials<4>(partials, nb, v);
+  const double pkp = v[0], c_tr = v[1], c_tt = v[2], actual = v[3];
+  const double a = rnd(st, -st->fs_rsq / pkp);
+  const double rsq = actual + 2.0 * a * c_tr + a * a * c_tt;
+  const int it = st->fs_iter + 1;
+  const double size_r = rnd(st, rsq) / st->src_norm;
+  const double b = rnd(st, rnd(st, rsq) / actual);
+  const bool conv = st->rsqmin <= 0 || st->rsqmin > size_r;
+  const bool stop = it % st->niter == 0 || conv;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iteration = it;
+    st->oldrsq = actual;
+    st->pkp = pkp;
+    st->a = a;
+    st->rsq = rsq;
+    st->size_r = size_r;
+    st->final_rsq = size_r;
+    st->b = b;
+    if (stop) *gate = 0;
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  using C = typename Spin<T>::C;
+  const C ac = (C)a, bc = (C)b;
+  cplx<C> xv[3], pv[3], rv[3], tv[3];
+  Spin<T>::load(x, i, xv);
+  Spin<T>::load(p, i, pv);
+  Spin<T>::load(r, i, rv);
+  Spin<T>::load(ttt, i, tv);
+  for (int c = 0; c < 3; ++c) {
+    xv[c].re = xv[c].re + ac * pv[c].re;
+    xv[c].im = xv[c].im + ac * pv[c].im;
+    rv[c].re = rv[c].re + ac * tv[c].re;
+    rv[c].im = rv[c].im + ac * tv[c].im;
+  }
+  Spin<T>::store(x, i, xv);
+  Spin<T>::store(r, i, rv);
+  if (stop) return;
   for (int c = 0; c < 3; ++c) {
     pv[c].re = rv[c].re + bc * pv[c].re;
     pv[c].im = rv[c].im + bc * pv[c].im;
@@ -302,6 +364,14 @@ void blas_cg_step_c(T *x, T *p, T *r, const T *ttt, SolverState *st, int *gate, 
   HIP_CHECK(hipGetLastError());
 }
 
+template <typename T>
+void blas_cg_step_fs(T *x, T *p, T *r, const T *ttt, SolverState *st, int *gate, int nb_pkp) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_cg_step_fs<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, x, p, r, ttt, vh, st,
+                     ctx().partials, nb_pkp, gate);
+  HIP_CHECK(hipGetLastError());
+}
+
 template <typename T> void blas_ms_update_r(T *r, const T *ttt, SolverState *st, const int *gate) {
   const int vh = ctx().g.vh;
   const ReduceCtl rc = reduce_ctl(st, SOP_MS_RSQ2);
@@ -337,6 +407,7 @@ void set_int(int *p, int v) {
   template void blas_norm2<T>(const T *, SolverState *, int);                                  \
   template void blas_cg_update<T>(T *, const T *, T *, const T *, SolverState *, const int *); \
   template void blas_cg_step_c<T>(T *, T *, T *, const T *, SolverState *, int *, int);         \
+  template void blas_cg_step_fs<T>(T *, T *, T *, const T *, SolverState *, int *, int);        \
   template void blas_cg_axpy<T>(const T *, T *, const SolverState *, const int *);             \
   template void blas_ms_update_r<T>(T *, const T *, SolverState *, const int *);               \
   template void blas_ms_update_pm<T>(T *const *, T *const *, const T *, int,                   \

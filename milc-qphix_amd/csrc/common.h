@@ -88,6 +88,11 @@ void sync_if_needed();
 void *ws_get(size_t bytes);  // grows; valid until the next ws_get with more bytes
 void dev_free(void *p);
 void *dev_alloc(size_t bytes);
+// For optional buffers the caller can do without: null on failure, nothing
+// recorded (so no abort under abort-on-error).
+void *dev_try_alloc(size_t bytes);
+int fewsums_mode();
+void set_fewsums_mode(int mode);
 
 // ---------------------------------------------------------------------------
 // Device field storage.
@@ -213,14 +218,17 @@ struct alignas(16) HalfSite {
 enum DslashEpilogue {
   EPI_PLAIN = 0,  // out = D in
   EPI_PKP = 1,    // out = D in + s*aux ; reduce aux.out
-  EPI_RESTART = 2 // r = b + (D in + s*aux) ; p = r ; reduce |r|^2 (+ rel)
+  EPI_RESTART = 2, // r = b + (D in + s*aux) ; p = r ; reduce |r|^2 (+ rel)
+  EPI_PKP_FS = 3   // out = D in + s*aux ; reduce aux.out, r.out, out.out, r.r
+                   // (r read-only); block 0 snapshots the gate and scalars
+                   // for the single-reduction CG step that follows
 };
 
 struct DslashExtra {
   double s = 0;                // scalar of the epilogue (-msq_x4 or -sigma)
   const void *aux = nullptr;   // p (PKP) or x (RESTART)
   const void *b = nullptr;     // RESTART: source
-  void *r = nullptr;           // RESTART: residual out
+  void *r = nullptr;           // RESTART: residual out; PKP_FS: residual in
   void *p = nullptr;           // RESTART: direction out
   void *state = nullptr;       // device SolverState updated by the last block
   int scalar_op = 0;           // ScalarOp run by the last block
@@ -241,13 +249,15 @@ void launch_dslash_q16(const LinksT<Q16> &L, int cb, float *out, const float *in
 void launch_dslash_q16_half(const LinksT<Q16> &L, int cb, HalfSite *out_h, float *out_f,
                             const HalfSite *in, int epi, const DslashExtra &ex);
 
-// FP32 copy of FP64 links in the same layout (fields.hip)
-void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf);
-// 16-bit fixed-point copy of FP64 links (fields.hip)
-void links_to_q16(const LinksT<double> &Ld, LinksT<Q16> &Lq);
-// The lazily built sloppy copies of a D3 link set (fields.hip)
-const LinksT<float> &sloppy_links(LinksT<double> *L);
-const LinksT<Q16> &sloppy_links16(LinksT<double> *L);
+// FP32 copy of FP64 links in the same layout (fields.hip); false (error
+// recorded, nothing left allocated) when device memory runs out
+bool links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf);
+// 16-bit fixed-point copy of FP64 links (fields.hip); false as above
+bool links_to_q16(const LinksT<double> &Ld, LinksT<Q16> &Lq);
+// The lazily built sloppy copies of a D3 link set (fields.hip); null (error
+// recorded) when they cannot be allocated
+const LinksT<float> *sloppy_links(LinksT<double> *L);
+const LinksT<Q16> *sloppy_links16(LinksT<double> *L);
 // Link bits of the mixed solvers' sloppy operator: 32 (default) or 16.
 int sloppy_link_bits();
 
@@ -265,8 +275,10 @@ struct BatchGraph {
   BatchGraph() = default;
   BatchGraph(const BatchGraph &) = delete;
   BatchGraph &operator=(const BatchGraph &) = delete;
-  template <typename F> void run(F &&enqueue) {
-    if (!ctx().graphs) {
+  // use_graph = false: plain launches whatever ctx().graphs says (capture
+  // and instantiation allocate; the no-allocation solver must not)
+  template <typename F> void run(F &&enqueue, bool use_graph = true) {
+    if (!ctx().graphs || !use_graph) {
       enqueue();
       return;
     }
@@ -289,5 +301,11 @@ struct BatchGraph {
 
 // Second workspace, for drivers that call other solvers while holding theirs.
 void *ws2_get(size_t bytes);
+
+// Symanzik force step mom <- mom - eps TA(W) queued on the library stream
+// (gauge_force.hip); info->status = QPHIX_SUCCESS once queued.
+template <typename T>
+void gauge_force_apply(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_gauge_coeffs_t *c, double eps);
+
 
 } // namespace qphix

@@ -36,12 +36,46 @@ void sync_if_needed() {
   if (g_ctx.sync_calls) HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
 }
 
+// Allocation-failure injection for the error-path tests: after `n` more
+// successful allocations every dev_alloc fails as if device memory were
+// exhausted (negative: never).
+static long g_fail_alloc_after = -1;
+
 void *dev_alloc(size_t bytes) {
   void *p = nullptr;
   if (bytes == 0) return nullptr;
+  if (g_fail_alloc_after == 0) {
+    fail(__FILE__, __LINE__, "hipMalloc failed (device memory exhausted?) [injected]");
+    return nullptr;
+  }
+  if (g_fail_alloc_after > 0) --g_fail_alloc_after;
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
     fail(__FILE__, __LINE__, "hipMalloc failed (device memory exhausted?)");
+    return nullptr;
+  }
+  return p;
+}
+
+// Single-reduction CG steps: bit 0 the one-precision CG, bit 1 the mixed
+// solver's sloppy steps.  QPHIX_FEWSUMS in the environment sets the initial
+// value; QPHIX_MI355X_set_fewsums wins.
+static int g_fewsums = -1;
+int fewsums_mode() {
+  if (g_fewsums < 0) {
+    const char *e = std::getenv("QPHIX_FEWSUMS");
+    g_fewsums = e ? (std::atoi(e) & 3) : 2;
+  }
+  return g_fewsums;
+}
+void set_fewsums_mode(int mode) { g_fewsums = mode & 3; }
+
+void *dev_try_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0 || g_fail_alloc_after == 0) return nullptr;
+  if (g_fail_alloc_after > 0) --g_fail_alloc_after;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();  // clear the sticky error: the caller falls back
     return nullptr;
   }
   return p;
@@ -200,6 +234,8 @@ void QPHIX_MI355X_clear_error(void) {
 }
 
 void QPHIX_MI355X_set_sync(int on) { g_ctx.sync_calls = on != 0; }
+void QPHIX_MI355X_debug_fail_alloc(int after) { g_fail_alloc_after = after; }
+void QPHIX_MI355X_set_fewsums(int mode) { qphix::set_fewsums_mode(mode); }
 void QPHIX_MI355X_set_graphs(int on) { g_ctx.graphs = on != 0; }
 
 void QPHIX_MI355X_sync(void) {

@@ -26,6 +26,20 @@ __device__ __forceinline__ void matvec_acc(const cplx<C> *U, const cplx<C> *v, c
   }
 }
 
+// Per-site term of the relative residue (ks_blas_utils_c.cpp:110-149): each
+// colour's |v_c|^2 is formed in the vector type and widened to double, as the
+// reference's calc_soalen_rel_residue does (:118-121).
+template <typename C> __device__ __forceinline__ double site_norm2_rounded(const cplx<C> *v) {
+  double s = 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s += (double)(C)(v[c].re * v[c].re + v[c].im * v[c].im);
+  return s;
+}
+// num / den with the reference's special cases: r = 0 skipped, x = 0 counts 1.
+__device__ __forceinline__ double rel_term(double num, double den) {
+  return num == 0 ? 0.0 : (den == 0 ? 1.0 : num / den);
+}
+
 // Third row of a link from its first two: s * conj(r0 x r1).
 template <typename C>
 __device__ __forceinline__ void recon_row2(cplx<C> *U, C s) {

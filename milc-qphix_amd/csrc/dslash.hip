@@ -16,6 +16,9 @@
 //   EPI_RESTART r = b + D in + s*x, p = r, reduce |r|^2 and the relative residue
 //                                                   (calc_rsq, ks_d_congrad_fn.cpp:82-99;
 //                                                    relative_residue, ks_blas_utils_c.cpp:110-149)
+//   EPI_PKP_FS  out = D in + s*p, reduce p.out, r.out, out.out and r.r: the one
+//               reduction of a single-reduction (FEWSUMS) CG step
+//                                                   (ks_d_congrad_fn.cpp:150-158, 450-541)
 #include "blas.h"
 #include "dslash_kernel.h"
 
@@ -41,14 +44,17 @@ static void dispatch(const DslashKArgs<TL, SI, SO> &a, int recon, int epi, bool 
   if (recon == 12) {
     if (epi == EPI_PLAIN) launch_one<TL, 12, SI, SO, EPI_PLAIN>(a, nb);
     else if (epi == EPI_PKP) launch_one<TL, 12, SI, SO, EPI_PKP>(a, nb);
+    else if (epi == EPI_PKP_FS) launch_one<TL, 12, SI, SO, EPI_PKP_FS>(a, nb);
     else launch_one<TL, 12, SI, SO, EPI_RESTART>(a, nb);
   } else {
     if (epi == EPI_PLAIN) launch_one<TL, 18, SI, SO, EPI_PLAIN>(a, nb);
     else if (epi == EPI_PKP) launch_one<TL, 18, SI, SO, EPI_PKP>(a, nb);
+    else if (epi == EPI_PKP_FS) launch_one<TL, 18, SI, SO, EPI_PKP_FS>(a, nb);
     else launch_one<TL, 18, SI, SO, EPI_RESTART>(a, nb);
   }
   HIP_CHECK(hipGetLastError());
-  if (consumer) return;  // the next kernel sums the partials itself
+  // the single-reduction step always sums its partials in the next kernel
+  if (consumer || epi == EPI_PKP_FS) return;  // the next kernel sums the partials itself
   if (epi == EPI_PKP) reduce_finish(1, nb, a.rc, a.gate);
   else if (epi == EPI_RESTART) reduce_finish(2, nb, a.rc, a.gate);
 }

@@ -109,7 +109,21 @@ __device__ __forceinline__ void apply_pair(const typename vec16<TL>::type *base,
 
 template <typename TL, int NL, typename SI, typename SO, int EPI, int OPT>
 __global__ void __launch_bounds__(DSLASH_BLOCK) QPHIX_DSLASH_OCC k_dslash(const DslashKArgs<TL, SI, SO> a) {
-  if (a.gate && *a.gate == 0) return;  // uniform over the grid
+  if (a.gate && *a.gate == 0) {  // uniform over the grid
+    if constexpr (EPI == EPI_PKP_FS) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) a.rc.state->fs_gate = 0;  // the step is off
+    }
+    return;
+  }
+  if constexpr (EPI == EPI_PKP_FS) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      SolverState *st = a.rc.state;
+      st->fs_gate = 1;
+      st->fs_iter = st->iteration;
+      st->fs_rsq = st->rsq;
+      st->fs_mxrsq = st->mx_rsq;
+    }
+  }
   using TC = typename link_compute<TL>::type;
   constexpr int NCH_F = link_nch<TL>(18), NCH_L = link_nch<TL>(NL);
   constexpr bool NTL = (OPT & OPT_NTL) != 0;
@@ -157,13 +171,33 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) QPHIX_DSLASH_OCC k_dslash(const 
       Spin<SO>::store(a.out, i, acc);
     }
     grid_reduce<1>(v, a.rc);
+  } else if constexpr (EPI == EPI_PKP_FS) {
+    // ks_d_congrad_fn.cpp:150-158 (FEWSUMS): with ttt = D in + s p, the sums
+    // p.ttt, Re<ttt|r>, <ttt|ttt> and |r|^2 in one reduction
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (valid) {
+      cplx<TC> p[3], r[3];
+      Spin<SI>::load(a.aux, i, p);
+      Spin<SO>::load(a.r, i, r);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        acc[c].re = acc[c].re + a.s * p[c].re;
+        acc[c].im = acc[c].im + a.s * p[c].im;
+        v[0] += (double)p[c].re * (double)acc[c].re + (double)p[c].im * (double)acc[c].im;
+        v[1] += (double)r[c].re * (double)acc[c].re + (double)r[c].im * (double)acc[c].im;
+        v[2] += (double)acc[c].re * (double)acc[c].re + (double)acc[c].im * (double)acc[c].im;
+        v[3] += (double)r[c].re * (double)r[c].re + (double)r[c].im * (double)r[c].im;
+      }
+      Spin<SO>::store(a.out, i, acc);
+    }
+    grid_reduce<4>(v, a.rc);
   } else {  // EPI_RESTART
     double v[2] = {0.0, 0.0};
     if (valid) {
       cplx<TC> x[3], b[3];
       Spin<SI>::load(a.aux, i, x);
       Spin<SI>::load(a.b, i, b);
-      double num = 0, den = 0;
+      double num = 0;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const TC tr = acc[c].re + a.s * x[c].re;
@@ -171,12 +205,11 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) QPHIX_DSLASH_OCC k_dslash(const 
         acc[c].re = b[c].re + tr;
         acc[c].im = b[c].im + ti;
         num += (double)acc[c].re * acc[c].re + (double)acc[c].im * acc[c].im;
-        den += (double)x[c].re * x[c].re + (double)x[c].im * x[c].im;
       }
       Spin<SO>::store(a.r, i, acc);
       Spin<SO>::store(a.p, i, acc);
       v[0] = num;
-      v[1] = num == 0 ? 0.0 : (den == 0 ? 1.0 : num / den);
+      v[1] = rel_term(site_norm2_rounded(acc), site_norm2_rounded(x));
     }
     grid_reduce<2>(v, a.rc);
   }

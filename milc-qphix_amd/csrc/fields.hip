@@ -283,7 +283,7 @@ template void links_free<double>(LinksT<double> *);
 template void links_free<float>(LinksT<float> *);
 template void links_free<Q16>(LinksT<Q16> *);
 
-void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf) {
+bool links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf) {
   const int vh = ctx().g.vh;
   Lf.recon = Ld.recon;
   Lf.parity = Ld.parity;
@@ -291,20 +291,25 @@ void links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf) {
     if (!Ld.fat[cb]) continue;
     Lf.fat[cb] = (float *)dev_alloc(sizeof(float) * 4 * 36 * link_sites(vh));
     Lf.lng[cb] = (float *)dev_alloc(sizeof(float) * 4 * 2 * Ld.recon * link_sites(vh));
+    if (Ld.lsign[cb]) Lf.lsign[cb] = (uint8_t *)dev_alloc((size_t)vh);
+    if (!Lf.fat[cb] || !Lf.lng[cb] || (Ld.lsign[cb] && !Lf.lsign[cb])) {  // error recorded
+      HIP_CHECK(hipStreamSynchronize(ctx().stream));  // earlier parity's copies done
+      links_free(&Lf);
+      return false;
+    }
     hipLaunchKernelGGL(k_links_to_float, dim3(nblk(vh)), dim3(256), 0, ctx().stream, Lf.fat[cb],
                        Ld.fat[cb], vh, 18);
     hipLaunchKernelGGL(k_links_to_float, dim3(nblk(vh)), dim3(256), 0, ctx().stream, Lf.lng[cb],
                        Ld.lng[cb], vh, Ld.recon == 12 ? 12 : 18);
-    if (Ld.lsign[cb]) {
-      Lf.lsign[cb] = (uint8_t *)dev_alloc((size_t)vh);
+    if (Ld.lsign[cb])
       HIP_CHECK(hipMemcpyAsync(Lf.lsign[cb], Ld.lsign[cb], (size_t)vh, hipMemcpyDeviceToDevice,
                                ctx().stream));
-    }
   }
   HIP_CHECK(hipGetLastError());
+  return true;
 }
 
-void links_to_q16(const LinksT<double> &Ld, LinksT<Q16> &Lq) {
+bool links_to_q16(const LinksT<double> &Ld, LinksT<Q16> &Lq) {
   const int vh = ctx().g.vh;
   Lq.recon = Ld.recon;
   Lq.parity = Ld.parity;
@@ -313,33 +318,46 @@ void links_to_q16(const LinksT<double> &Ld, LinksT<Q16> &Lq) {
     if (!Ld.fat[cb]) continue;
     Lq.fat[cb] = (Q16 *)dev_alloc(16 * 4 * (size_t)link_nch<Q16>(18) * link_sites(vh));
     Lq.lng[cb] = (Q16 *)dev_alloc(16 * 4 * (size_t)link_nch<Q16>(ncl) * link_sites(vh));
+    if (Ld.lsign[cb]) Lq.lsign[cb] = (uint8_t *)dev_alloc((size_t)vh);
+    if (!Lq.fat[cb] || !Lq.lng[cb] || (Ld.lsign[cb] && !Lq.lsign[cb])) {  // error recorded
+      HIP_CHECK(hipStreamSynchronize(ctx().stream));  // earlier parity's copies done
+      links_free(&Lq);
+      return false;
+    }
     hipLaunchKernelGGL(k_links_to_q16, dim3(nblk(vh)), dim3(256), 0, ctx().stream,
                        reinterpret_cast<uint4 *>(Lq.fat[cb]), Ld.fat[cb], vh, 18);
     hipLaunchKernelGGL(k_links_to_q16, dim3(nblk(vh)), dim3(256), 0, ctx().stream,
                        reinterpret_cast<uint4 *>(Lq.lng[cb]), Ld.lng[cb], vh, ncl);
-    if (Ld.lsign[cb]) {
-      Lq.lsign[cb] = (uint8_t *)dev_alloc((size_t)vh);
+    if (Ld.lsign[cb])
       HIP_CHECK(hipMemcpyAsync(Lq.lsign[cb], Ld.lsign[cb], (size_t)vh, hipMemcpyDeviceToDevice,
                                ctx().stream));
-    }
   }
   HIP_CHECK(hipGetLastError());
+  return true;
 }
 
-const LinksT<float> &sloppy_links(LinksT<double> *L) {
+const LinksT<float> *sloppy_links(LinksT<double> *L) {
   if (!L->sloppy) {
-    L->sloppy = new LinksT<float>;
-    links_to_float(*L, *L->sloppy);
+    auto *s = new LinksT<float>;
+    if (!links_to_float(*L, *s)) {
+      delete s;
+      return nullptr;
+    }
+    L->sloppy = s;
   }
-  return *L->sloppy;
+  return L->sloppy;
 }
 
-const LinksT<Q16> &sloppy_links16(LinksT<double> *L) {
+const LinksT<Q16> *sloppy_links16(LinksT<double> *L) {
   if (!L->sloppy16) {
-    L->sloppy16 = new LinksT<Q16>;
-    links_to_q16(*L, *L->sloppy16);
+    auto *s = new LinksT<Q16>;
+    if (!links_to_q16(*L, *s)) {
+      delete s;
+      return nullptr;
+    }
+    L->sloppy16 = s;
   }
-  return *L->sloppy16;
+  return L->sloppy16;
 }
 
 static int g_sloppy_link_bits = 32;
@@ -379,7 +397,7 @@ static typename Handles<T>::V *create_V(const T *src, int evenodd) {
   T *stage = nullptr;
   if (src) {
     stage = (T *)dev_alloc(sizeof(T) * nv);
-    QPHIX_REQUIRE(stage, "create_V: out of device memory", nullptr);
+    if (!stage) return nullptr;  // error recorded
   }
   auto *V = new typename Handles<T>::V;
   V->parity = evenodd;
@@ -416,7 +434,7 @@ template <typename T> static void extract_V(T *dest, const typename Handles<T>::
   const int vh = ctx().g.vh;
   const size_t nv = 6 * (size_t)vh;
   T *stage = (T *)dev_alloc(sizeof(T) * nv);
-  QPHIX_REQUIRE(stage, "extract_V: out of device memory");
+  if (!stage) return;  // error recorded
   for (int cb = 0; cb < 2; ++cb)
     if (V->half[cb]) {
       hipLaunchKernelGGL(k_spin_to_raw<T>, dim3((vh + 255) / 256), dim3(256), 0, ctx().stream, stage,
@@ -459,6 +477,14 @@ static typename Handles<T>::L *create_L(const T *fat, const T *lng, const T *fat
     h2d(stage + nraw, lngback + off, sizeof(T) * nraw);
     HIP_CHECK(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), ctx().stream));
     signs[cb] = (uint8_t *)dev_alloc((size_t)vh);
+    if (!signs[cb]) {  // error recorded
+      dev_free(signs[0]);
+      dev_free(signs[1]);
+      dev_free(stage);
+      dev_free(dmax);
+      delete L;
+      return nullptr;
+    }
     hipLaunchKernelGGL(k_recon_check<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, stage,
                        stage + nraw, vh, signs[cb], dmax);
     unsigned long long bits = 0;
@@ -479,6 +505,16 @@ static typename Handles<T>::L *create_L(const T *fat, const T *lng, const T *fat
     h2d(stage + 3 * nraw, lngback + off, sizeof(T) * nraw);
     L->fat[cb] = (T *)dev_alloc(sizeof(T) * 144 * link_sites(vh));
     L->lng[cb] = (T *)dev_alloc(sizeof(T) * 2 * 2 * ncl * 4 * link_sites(vh));
+    if (!L->fat[cb] || !L->lng[cb]) {  // error recorded
+      sync();
+      dev_free(signs[0]);
+      dev_free(signs[1]);
+      dev_free(stage);
+      dev_free(dmax);
+      links_free<T>(L);
+      delete L;
+      return nullptr;
+    }
     hipLaunchKernelGGL(k_import_pairs<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, L->fat[cb],
                        stage, stage + nraw, vh, 9);
     hipLaunchKernelGGL(k_import_pairs<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, L->lng[cb],
@@ -501,6 +537,7 @@ static void extract_L(T *fat, T *lng, const typename Handles<T>::L *L, int eveno
   const int vh = ctx().g.vh;
   const size_t nraw = 72 * (size_t)vh;
   T *stage = (T *)dev_alloc(sizeof(T) * 2 * nraw);
+  if (!stage) return;  // error recorded
   for (int cb = 0; cb < 2; ++cb) {
     if (!(evenodd & (cb ? QPHIX_ODD : QPHIX_EVEN))) continue;
     if (!L->fat[cb]) continue;
@@ -526,6 +563,7 @@ template <typename T> static void backcheck(const LinksT<T> *L, double out[4]) {
   const Geometry &gg = ctx().g;
   Geo g{gg.nxh, gg.ny, gg.nz, gg.nt, gg.vh, gg.nxy, gg.nxyz};
   unsigned long long *d = (unsigned long long *)dev_alloc(sizeof(unsigned long long) * 4);
+  if (!d) return;  // error recorded
   HIP_CHECK(hipMemsetAsync(d, 0, sizeof(unsigned long long) * 4, ctx().stream));
   const int ncl = L->recon == 12 ? 6 : 9;
   for (int cb = 0; cb < 2; ++cb)
@@ -617,6 +655,13 @@ template <typename T> static typename Handles<T>::G *create_G(const T *raw, int 
   G->parity = evenodd == QPHIX_EVENODD ? QPHIX_EVEN : evenodd;
   for (int cb = 0; cb < 2; ++cb) {
     G->u[cb] = (T *)dev_alloc(sizeof(T) * n);
+    if (!G->u[cb]) {  // error recorded
+      sync();
+      dev_free(G->u[0]);
+      dev_free(stage);
+      delete G;
+      return nullptr;
+    }
     h2d(stage, raw + (cb ? n : 0), sizeof(T) * n);
     hipLaunchKernelGGL(k_gauge_in<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, G->u[cb], stage, vh);
     HIP_CHECK(hipGetLastError());
@@ -626,7 +671,7 @@ template <typename T> static typename Handles<T>::G *create_G(const T *raw, int 
   // Two-row copy for the force kernel when every link is SU(3) to rounding
   // (the exact links stay in u for extract_G).
   bool su3 = g_gauge_recon_mode != 18;
-  unsigned long long *dmax = su3 ? (unsigned long long *)dev_alloc(sizeof(unsigned long long)) : nullptr;
+  unsigned long long *dmax = su3 ? (unsigned long long *)dev_try_alloc(sizeof(unsigned long long)) : nullptr;
   if (!dmax) su3 = false;
   for (int cb = 0; cb < 2 && su3; ++cb) {
     HIP_CHECK(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), ctx().stream));
@@ -641,7 +686,7 @@ template <typename T> static typename Handles<T>::G *create_G(const T *raw, int 
   }
   dev_free(dmax);
   if (su3) {
-    for (int cb = 0; cb < 2; ++cb) G->u12[cb] = (T *)dev_alloc(sizeof(T) * 48 * (size_t)vh);
+    for (int cb = 0; cb < 2; ++cb) G->u12[cb] = (T *)dev_try_alloc(sizeof(T) * 48 * (size_t)vh);
     if (!G->u12[0] || !G->u12[1]) {  // no room: the force kernel reads the full links
       dev_free(G->u12[0]);
       dev_free(G->u12[1]);
@@ -694,11 +739,19 @@ template <typename T> static typename Handles<T>::F *create_F(const T *raw, int 
                 "create_F: bad parity", nullptr);
   const int vh = ctx().g.vh;
   const size_t n = 72 * (size_t)vh;
+  T *stage = (T *)dev_alloc(sizeof(T) * n);
+  if (!stage) return nullptr;  // error recorded
   auto *F = new typename Handles<T>::F;
   F->parity = evenodd == QPHIX_EVENODD ? QPHIX_EVEN : evenodd;
-  T *stage = (T *)dev_alloc(sizeof(T) * n);
   for (int cb = 0; cb < 2; ++cb) {
     F->h[cb] = (T *)dev_alloc(sizeof(T) * 32 * (size_t)vh);
+    if (!F->h[cb]) {  // error recorded
+      sync();
+      dev_free(F->h[0]);
+      dev_free(stage);
+      delete F;
+      return nullptr;
+    }
     HIP_CHECK(hipMemsetAsync(F->h[cb], 0, sizeof(T) * 32 * (size_t)vh, ctx().stream));
     if (!(evenodd & (cb ? QPHIX_ODD : QPHIX_EVEN))) continue;
     h2d(stage, raw + (cb ? n : 0), sizeof(T) * n);
@@ -718,6 +771,7 @@ template <typename T> static void extract_F(T *raw, const typename Handles<T>::F
   const int vh = ctx().g.vh;
   const size_t n = 72 * (size_t)vh;
   T *stage = (T *)dev_alloc(sizeof(T) * n);
+  if (!stage) return;  // error recorded
   for (int cb = 0; cb < 2; ++cb) {
     if (!(evenodd & (cb ? QPHIX_ODD : QPHIX_EVEN))) continue;
     hipLaunchKernelGGL(k_mom_out<T>, dim3(nblk(4 * vh)), dim3(256), 0, ctx().stream, stage,

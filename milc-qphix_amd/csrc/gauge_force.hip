@@ -481,9 +481,11 @@ template <typename T> static bool gf_step_tables(GFArgs<T> &a) {
   return true;
 }
 
+// Queues the force kernel (no synchronisation); info->status is set to
+// QPHIX_SUCCESS once it is queued.
 template <typename T>
-static void gauge_force(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_gauge_coeffs_t *c,
-                        double eps) {
+void gauge_force_apply(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_gauge_coeffs_t *c, double eps) {
+  info->status = QPHIX_FAIL;
   using namespace std::chrono;
   const double t0 = duration<double>(steady_clock::now().time_since_epoch()).count();
   const Geometry &gg = ctx().g;
@@ -516,10 +518,22 @@ static void gauge_force(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_ga
   else
     hipLaunchKernelGGL((k_gauge_force<T, 9, 1>), grid, dim3(GF_BLOCK), 0, ctx().stream, a);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(ctx().stream));
   info->status = QPHIX_SUCCESS;
   info->final_sec = duration<double>(steady_clock::now().time_since_epoch()).count() - t0;
   info->final_flop = 61092.0;  // the reference's constant, gauge_force_imp.cpp:197
+}
+template void gauge_force_apply<double>(QPHIX_info_t *, GaugeT<double> *, ForceT<double> *,
+                                        QPHIX_gauge_coeffs_t *, double);
+template void gauge_force_apply<float>(QPHIX_info_t *, GaugeT<float> *, ForceT<float> *,
+                                       QPHIX_gauge_coeffs_t *, double);
+
+template <typename T>
+static void gauge_force(QPHIX_info_t *info, GaugeT<T> *G, ForceT<T> *F, QPHIX_gauge_coeffs_t *c, double eps) {
+  using namespace std::chrono;
+  const double t0 = duration<double>(steady_clock::now().time_since_epoch()).count();
+  gauge_force_apply<T>(info, G, F, c, eps);
+  HIP_CHECK(hipStreamSynchronize(ctx().stream));
+  info->final_sec = duration<double>(steady_clock::now().time_since_epoch()).count() - t0;
 }
 
 } // namespace qphix

@@ -20,52 +20,6 @@
 
 namespace qphix {
 
-// x_s += alpha p ; r_s += alpha Ap ; reduce |r_s|^2
-template <typename P>
-__global__ void __launch_bounds__(BLAS_BLOCK)
-    k_mx_step(float *xs, const P *p, float *rs, const float *Ap, int vh, ReduceCtl rc,
-              const int *gate) {
-  if (*gate == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double s[1] = {0.0};
-  if (i < vh) {
-    const float al = (float)rc.state->mx_alpha;
-    cplx<float> xv[3], pv[3], rv[3], av[3];
-    Spin<float>::load(xs, i, xv);
-    Spin<P>::load(p, i, pv);
-    Spin<float>::load(rs, i, rv);
-    Spin<float>::load(Ap, i, av);
-    for (int c = 0; c < 3; ++c) {
-      xv[c].re += al * pv[c].re;
-      xv[c].im += al * pv[c].im;
-      rv[c].re += al * av[c].re;
-      rv[c].im += al * av[c].im;
-      s[0] += (double)rv[c].re * rv[c].re + (double)rv[c].im * rv[c].im;
-    }
-    Spin<float>::store(xs, i, xv);
-    Spin<float>::store(rs, i, rv);
-  }
-  grid_reduce<1>(s, rc);
-}
-
-// p = r_s + beta p
-template <typename P>
-__global__ void __launch_bounds__(BLAS_BLOCK)
-    k_mx_axpy(const float *rs, P *p, int vh, const SolverState *st, const int *gate) {
-  if (*gate == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= vh) return;
-  const float be = (float)st->mx_beta;
-  cplx<float> rv[3], pv[3];
-  Spin<float>::load(rs, i, rv);
-  Spin<P>::load(p, i, pv);
-  for (int c = 0; c < 3; ++c) {
-    pv[c].re = rv[c].re + be * pv[c].re;
-    pv[c].im = rv[c].im + be * pv[c].im;
-  }
-  Spin<P>::store(p, i, pv);
-}
-
 // The sloppy step with consumer-side reductions (the arithmetic of SOP_MX_PAP
 // and k_mx_step): every block derives alpha = -|r|^2 / p.(-Mp) from the
 // preceding Dslash's partials; block 0 records pkp, alpha, the iteration and
@@ -136,6 +90,63 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   cplx<float> rv[3], pv[3];
   Spin<float>::load(rs, i, rv);
   Spin<P>::load(p, i, pv);
+  for (int c = 0; c < 3; ++c) {
+    pv[c].re = rv[c].re + be * pv[c].re;
+    pv[c].im = rv[c].im + be * pv[c].im;
+  }
+  Spin<P>::store(p, i, pv);
+}
+
+// The sloppy step as a single-reduction (FEWSUMS) step: the preceding
+// EPI_PKP_FS Dslash reduced pkp = p.Ap (Ap = -Mp), c_tr = r.Ap, c_tt = Ap.Ap
+// and rr = |r|^2, and snapshotted the gate, the iteration and |r|^2 (mx_rsq:
+// the previous step's estimate, or the true value after a reliable update).
+// alpha = -|r|^2 / pkp; x_s += alpha p; r_s += alpha Ap; the new |r|^2 follows
+// as rr + 2 alpha c_tr + alpha^2 c_tt (ks_d_congrad_fn.cpp:531); beta = that /
+// rr; the reliable-update test of k_mx_axpy_c; p = r_s + beta p unless an
+// update is due.  One pass over the vectors and no second reduction per step.
+template <typename P>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_mx_step_fs(float *xs, P *p, float *rs, const float *Ap, int vh, SolverState *st, const double *partials,
+                 int nb, int *gate) {
+  if (st->fs_gate == 0) return;  // written by the preceding kernel: uniform
+  double v[4];
+  sum_partials<4>(partials, nb, v);
+  const double pkp = v[0], c_tr = v[1], c_tt = v[2], rr = v[3];
+  const double alpha = -st->fs_mxrsq / pkp;
+  const double rnew = rr + 2.0 * alpha * c_tr + alpha * alpha * c_tt;
+  const double beta = rnew / rr;
+  const int it = st->fs_iter + 1;
+  const double r = sqrt(fmax(rnew, 0.0));
+  const double maxr = fmax(st->mx_maxr, r);  // idempotent: block 0 may have stored it already
+  const bool update = !(rnew > 0) || r < st->mx_delta * maxr || rnew < st->rsqstop || it >= st->max_cg;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pkp = pkp;
+    st->iteration = it;
+    st->mx_alpha = alpha;
+    st->mx_rsq_old = rr;
+    st->mx_beta = beta;
+    st->mx_rsq = rnew;
+    st->mx_maxr = maxr;
+    if (update) *gate = 0;
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  const float al = (float)alpha, be = (float)beta;
+  cplx<float> xv[3], pv[3], rv[3], av[3];
+  Spin<float>::load(xs, i, xv);
+  Spin<P>::load(p, i, pv);
+  Spin<float>::load(rs, i, rv);
+  Spin<float>::load(Ap, i, av);
+  for (int c = 0; c < 3; ++c) {
+    xv[c].re += al * pv[c].re;
+    xv[c].im += al * pv[c].im;
+    rv[c].re += al * av[c].re;
+    rv[c].im += al * av[c].im;
+  }
+  Spin<float>::store(xs, i, xv);
+  Spin<float>::store(rs, i, rv);
+  if (update) return;
   for (int c = 0; c < 3; ++c) {
     pv[c].re = rv[c].re + be * pv[c].re;
     pv[c].im = rv[c].im + be * pv[c].im;
@@ -234,67 +245,52 @@ static void write_state(SolverState *d, const SolverState &h) {
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
 }
 
-// Sloppy operator application Ap = -(4m^2 - D D) p, reducing p.Ap.
-// The p.Ap partials are left to the following k_mx_step_c (consumer_finalize).
-static void sloppy_apply(const LinksT<float> &Lf, int parity, float *p, float *tmp, float *Ap,
-                         double s, SolverState *st, const int *gate) {
+// Sloppy operator application Ap = -(4m^2 - D D) p, reducing p.Ap (and, for
+// a single-reduction step, r.Ap, Ap.Ap and |r|^2 with r = rs).  The partials
+// are left to the following step kernel (consumer_finalize).
+static DslashExtra pkp_extra(double s, const void *p, float *rs, SolverState *st, const int *gate) {
+  DslashExtra e;
+  e.gate = gate;
+  e.s = s;
+  e.aux = p;
+  e.r = rs;
+  e.state = st;
+  e.scalar_op = SOP_MX_PAP;
+  e.consumer_finalize = true;
+  return e;
+}
+
+static void sloppy_apply(const LinksT<float> &Lf, int parity, float *p, float *tmp, float *Ap, float *rs,
+                         double s, SolverState *st, const int *gate, int epi) {
   DslashExtra e1;
   e1.gate = gate;
   launch_dslash<float>(Lf, 1 - parity, tmp, p, EPI_PLAIN, e1);
-  DslashExtra e2;
-  e2.gate = gate;
-  e2.s = s;
-  e2.aux = p;
-  e2.state = st;
-  e2.scalar_op = SOP_MX_PAP;
-  e2.consumer_finalize = true;
-  launch_dslash<float>(Lf, parity, Ap, tmp, EPI_PKP, e2);
+  launch_dslash<float>(Lf, parity, Ap, tmp, epi, pkp_extra(s, p, rs, st, gate));
 }
 
-static void sloppy_apply(const LinksT<float> &Lf, int parity, HalfSite *p, HalfSite *tmp,
-                         float *Ap, double s, SolverState *st, const int *gate) {
+static void sloppy_apply(const LinksT<float> &Lf, int parity, HalfSite *p, HalfSite *tmp, float *Ap,
+                         float *rs, double s, SolverState *st, const int *gate, int epi) {
   DslashExtra e1;
   e1.gate = gate;
   launch_dslash_half(Lf, 1 - parity, tmp, nullptr, p, EPI_PLAIN, e1);
-  DslashExtra e2;
-  e2.gate = gate;
-  e2.s = s;
-  e2.aux = p;
-  e2.state = st;
-  e2.scalar_op = SOP_MX_PAP;
-  e2.consumer_finalize = true;
-  launch_dslash_half(Lf, parity, nullptr, Ap, tmp, EPI_PKP, e2);
+  launch_dslash_half(Lf, parity, nullptr, Ap, tmp, epi, pkp_extra(s, p, rs, st, gate));
 }
 
 // The same with 16-bit fixed-point links.
-static void sloppy_apply(const LinksT<Q16> &Lq, int parity, float *p, float *tmp, float *Ap, double s,
-                         SolverState *st, const int *gate) {
+static void sloppy_apply(const LinksT<Q16> &Lq, int parity, float *p, float *tmp, float *Ap, float *rs,
+                         double s, SolverState *st, const int *gate, int epi) {
   DslashExtra e1;
   e1.gate = gate;
   launch_dslash_q16(Lq, 1 - parity, tmp, p, EPI_PLAIN, e1);
-  DslashExtra e2;
-  e2.gate = gate;
-  e2.s = s;
-  e2.aux = p;
-  e2.state = st;
-  e2.scalar_op = SOP_MX_PAP;
-  e2.consumer_finalize = true;
-  launch_dslash_q16(Lq, parity, Ap, tmp, EPI_PKP, e2);
+  launch_dslash_q16(Lq, parity, Ap, tmp, epi, pkp_extra(s, p, rs, st, gate));
 }
 
-static void sloppy_apply(const LinksT<Q16> &Lq, int parity, HalfSite *p, HalfSite *tmp, float *Ap, double s,
-                         SolverState *st, const int *gate) {
+static void sloppy_apply(const LinksT<Q16> &Lq, int parity, HalfSite *p, HalfSite *tmp, float *Ap, float *rs,
+                         double s, SolverState *st, const int *gate, int epi) {
   DslashExtra e1;
   e1.gate = gate;
   launch_dslash_q16_half(Lq, 1 - parity, tmp, nullptr, p, EPI_PLAIN, e1);
-  DslashExtra e2;
-  e2.gate = gate;
-  e2.s = s;
-  e2.aux = p;
-  e2.state = st;
-  e2.scalar_op = SOP_MX_PAP;
-  e2.consumer_finalize = true;
-  launch_dslash_q16_half(Lq, parity, nullptr, Ap, tmp, EPI_PKP, e2);
+  launch_dslash_q16_half(Lq, parity, nullptr, Ap, tmp, epi, pkp_extra(s, p, rs, st, gate));
 }
 
 // P: storage of the sloppy vectors (float or HalfSite); LK: of the sloppy
@@ -306,13 +302,19 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
   const double t0 = wall_now();
   const int vh = ctx().g.vh;
   const size_t nv = 6 * (size_t)vh;
-  const LinksT<LK> &Lf = [&]() -> const LinksT<LK> & {
+  const LinksT<LK> *Lfp = [&]() -> const LinksT<LK> * {
     if constexpr (std::is_same<LK, Q16>::value) return sloppy_links16(L);
     else return sloppy_links(L);
   }();
   const size_t psz = sizeof(P) == sizeof(HalfSite) ? sizeof(HalfSite) * (size_t)vh : nv * sizeof(float);
-  char *ws = (char *)ws_get(2 * nv * sizeof(double) + 2 * psz + 3 * nv * sizeof(float) +
-                            sizeof(SolverState) + 1024);
+  char *ws = Lfp ? (char *)ws_get(2 * nv * sizeof(double) + 2 * psz + 3 * nv * sizeof(float) +
+                                   sizeof(SolverState) + 1024)
+                 : nullptr;
+  if (!ws) {  // out of device memory, error recorded
+    info->status = QPHIX_MEM_ERROR;
+    return 0;
+  }
+  const LinksT<LK> &Lf = *Lfp;
   double *r64 = (double *)ws, *tmp64 = r64 + nv;
   P *p = (P *)(tmp64 + nv);
   P *tmps = (P *)((char *)p + psz);
@@ -325,11 +327,25 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
   const int nbn = (int)((nv + BLAS_BLOCK - 1) / BLAS_BLOCK);
   const char *de = std::getenv("QPHIX_MIXED_DELTA");
   const int batch = 16;
+  const bool fs = (fewsums_mode() & 2) != 0;  // single-reduction sloppy steps
 
   SolverState hs{};
   hs.niter = inv->max;
   hs.max_cg = inv->max * inv->nrestart;
   hs.mx_delta = de ? std::atof(de) : 0.1;
+  // Stopping rule of the reference's CG (ks_d_congrad_fn.cpp:424-427): the
+  // squared relative true residual below resid and the relative residue
+  // (ks_blas_utils_c.cpp:110-149) below relresid, a test with a target <= 0
+  // counting as met.  Both are evaluated on the FP64 true residual at every
+  // reliable update; the sloppy steps only decide when to take one.
+  hs.rsqmin = res->resid;
+  hs.relrsqmin = res->relresid;
+  hs.vol = (double)ctx().g.nx * ctx().g.ny * ctx().g.nz * ctx().g.nt;
+  auto converged = [&](const SolverState &h) {
+    return (res->resid <= 0 || h.final_rsq < res->resid) &&
+           (res->relresid <= 0 || h.final_rel < res->relresid);
+  };
+  bool all_met = true;
   int todo[2], ntodo;
   if (inv->parity == QPHIX_EVENODD) { todo[0] = 0; todo[1] = 1; ntodo = 2; }
   else { todo[0] = inv->parity == QPHIX_EVEN ? 0 : 1; ntodo = 1; }
@@ -348,10 +364,11 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
     if (hs.src_norm == 0.0) {
       HIP_CHECK(hipMemsetAsync(x, 0, nv * sizeof(double), ctx().stream));
       hs.final_rsq = 0;
+      hs.final_rel = 0;
       iteration = 0;
       continue;
     }
-    hs.rsqstop = res->resid * hs.src_norm;
+    hs.rsqstop = res->resid > 0 ? res->resid * hs.src_norm : 0.0;
     write_state(dst, hs);
     bool first = true;
     while (true) {
@@ -369,7 +386,10 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
       e2.scalar_op = SOP_MX_TRUE;
       launch_dslash<double>(*L, parity, r64, tmp64, EPI_RESTART, e2);
       read_state(dst, hs);
-      if (hs.final_rsq < res->resid || hs.iteration >= hs.max_cg || !std::isfinite(hs.rsq)) break;
+      if (converged(hs) || hs.iteration >= hs.max_cg || !std::isfinite(hs.rsq)) {
+        if (!converged(hs)) all_met = false;
+        break;
+      }
       if (first) {
         hipLaunchKernelGGL(k_mx_init<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, r64, p, xs, vh);
         hipLaunchKernelGGL(k_mx_start, dim3(1), dim3(1), 0, ctx().stream, dst);
@@ -385,11 +405,17 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
       while (true) {
         steps.run([&]() {
           for (int k = 0; k < batch; ++k) {
-            sloppy_apply(Lf, parity, p, tmps, Ap, -msq_x4, dst, gate);
-            hipLaunchKernelGGL(k_mx_step_c<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs, Ap,
-                               vh, dst, ctx().partials, nb_dslash, partials_b(), gate);
-            hipLaunchKernelGGL(k_mx_axpy_c<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, vh, dst,
-                               partials_b(), nb, gate);
+            if (fs) {
+              sloppy_apply(Lf, parity, p, tmps, Ap, rs, -msq_x4, dst, gate, EPI_PKP_FS);
+              hipLaunchKernelGGL(k_mx_step_fs<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs,
+                                 Ap, vh, dst, ctx().partials, nb_dslash, gate);
+            } else {
+              sloppy_apply(Lf, parity, p, tmps, Ap, rs, -msq_x4, dst, gate, EPI_PKP);
+              hipLaunchKernelGGL(k_mx_step_c<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs,
+                                 Ap, vh, dst, ctx().partials, nb_dslash, partials_b(), gate);
+              hipLaunchKernelGGL(k_mx_axpy_c<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, vh,
+                                 dst, partials_b(), nb, gate);
+            }
           }
         });
         HIP_CHECK(hipGetLastError());
@@ -405,11 +431,14 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
   }
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
   res->final_rsq = hs.final_rsq;
-  res->final_rel = 0;
+  res->final_rel = hs.final_rel;
   res->size_r = hs.final_rsq;
-  res->size_relr = 0;
+  res->size_relr = hs.final_rel;
   res->final_iter = total;
   res->final_restart = updates;
+  // no counterpart in the reference (its CG leaves status alone): a solve cut
+  // off by max * nrestart before meeting its target reports failure
+  if (!all_met) info->status = QPHIX_FAIL;
   info->final_sec = wall_now() - t0;
   info->final_flop = 1187.0 * total * 2.0 * vh;
   return iteration;
@@ -456,12 +485,16 @@ double QPHIX_D3_asqtad_dslash_sloppy(QPHIX_D3_FermionLinksAsqtad *L, QPHIX_F3_Co
   HIP_CHECK(hipEventCreate(&e1));
   DslashExtra ex;
   if (link_bits == 16) {
-    const LinksT<Q16> &Lq = sloppy_links16(L);
+    const LinksT<Q16> *Lqp = sloppy_links16(L);
+    if (!Lqp) return -1.0;  // error recorded
+    const LinksT<Q16> &Lq = *Lqp;
     launch_dslash_q16(Lq, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
     HIP_CHECK(hipEventRecord(e0, ctx().stream));
     for (int r = 0; r < reps; ++r) launch_dslash_q16(Lq, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
   } else {
-    const LinksT<float> &Lf = sloppy_links(L);
+    const LinksT<float> *Lfp = sloppy_links(L);
+    if (!Lfp) return -1.0;  // error recorded
+    const LinksT<float> &Lf = *Lfp;
     launch_dslash<float>(Lf, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
     HIP_CHECK(hipEventRecord(e0, ctx().stream));
     for (int r = 0; r < reps; ++r) launch_dslash<float>(Lf, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);

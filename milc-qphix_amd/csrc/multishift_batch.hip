@@ -470,6 +470,7 @@ static double dslash_multi(LinksT<float> *L, QPHIX_F3_ColorVector **out, QPHIX_F
   const int vh = ctx().g.vh;
   const size_t nvr = 6 * (size_t)vh * NR;
   char *w = (char *)ws2_get(2 * nvr * sizeof(float) + NR * sizeof(SolverState) + 2 * NR * sizeof(float *) + 1024);
+  if (!w) return -1.0;  // out of device memory, error recorded
   float *pin = (float *)w, *pout = pin + nvr;
   SolverState *st = (SolverState *)(((uintptr_t)(pout + nvr) + 255) & ~(uintptr_t)255);
   float **ptrs = (float **)(st + NR);
@@ -581,11 +582,20 @@ int QPHIX_D3_asqtad_invert_multi_rhs(QPHIX_info_t *info, QPHIX_D3_FermionLinksAs
         QPHIX_REQUIRE(out[(size_t)r * nshift + j] && out[(size_t)r * nshift + j]->half[cb],
                       "asqtad_invert_multi_rhs: solution lacks a solved parity", 0);
     }
-    for (int j = 0; j < nshift; ++j) QPHIX_REQUIRE(res[(size_t)r * nshift + j], "asqtad_invert_multi_rhs: null res_arg", 0);
+    for (int j = 0; j < nshift; ++j) {
+      QPHIX_REQUIRE(res[(size_t)r * nshift + j], "asqtad_invert_multi_rhs: null res_arg", 0);
+      const double t = res[(size_t)r * nshift + j]->resid > 0 ? res[(size_t)r * nshift + j]->resid : res[0]->resid;
+      QPHIX_REQUIRE(t > 0, "asqtad_invert_multi_rhs: every solution needs a positive resid target", 0);
+    }
   }
   QPHIX_REQUIRE(rhs_block == 0 || rhs_block == 1 || rhs_block == 2 || rhs_block == 4 || rhs_block == 8,
                 "asqtad_invert_multi_rhs: rhs_block must be 0 (autotune), 1, 2, 4 or 8", 0);
-  const MrhsTune &tune = autotune_mrhs(sloppy_links(L));
+  const LinksT<float> *Lf = sloppy_links(L);
+  if (!Lf) {  // out of device memory, error recorded
+    info->status = QPHIX_MEM_ERROR;
+    return 0;
+  }
+  const MrhsTune &tune = autotune_mrhs(*Lf);
   if (tune.nr == 0) return 0;                 // workspace allocation failed (error recorded)
   int nb = rhs_block > 0 ? rhs_block : tune.nr;
   while (nb > 1 && nb / 2 >= nrhs) nb /= 2;   // no block wider than the sources need

@@ -22,6 +22,11 @@ struct SolverState {
   int single_coef;  // 1: scalars rounded to float as the F build's fptype does
   int converged, nshift, j_low, num_now, pad_;
   double offset_low;
+  // Single-reduction (FEWSUMS) steps: the p.Ap Dslash's block 0 copies the
+  // gate, the iteration and the |r|^2 estimates here before the fused update
+  // kernel runs, so that kernel reads values its own block 0 may overwrite.
+  int fs_gate, fs_iter;
+  double fs_rsq, fs_mxrsq;
   double shifts[MAXNDMASS], zeta_i[MAXNDMASS], zeta_im1[MAXNDMASS], zeta_ip1[MAXNDMASS];
   double beta_i[MAXNDMASS], beta_im1[MAXNDMASS], alpha[MAXNDMASS];
   int finished[MAXNDMASS];
@@ -187,6 +192,7 @@ __device__ inline void apply_scalar_op(SolverState *s, int op, const double *v) 
   case SOP_MX_TRUE:
     s->rsq = v[0];
     s->final_rsq = v[0] / s->src_norm;
+    if (s->relrsqmin > 0) s->final_rel = sqrt(2.0 * v[1] / s->vol);
     s->iteration++;
     break;
   case SOP_MSX_RSQ2: {

@@ -88,9 +88,16 @@ static double dslash_time(LinksT<T> *L, VectorT<T> *out, VectorT<T> *in, int par
 
 // ------------------------------------------------------------------ CG
 // QPHIX_*3_asqtad_invert, ks_d_congrad_fn.cpp:221-619.
+// ws_ext: a workspace of invert_ws_bytes() the caller provides (the
+// no-allocation entry point); then nothing is allocated and the iteration
+// batches are launched directly instead of through a captured graph.
+template <typename T> static size_t cg_ws_bytes() {
+  return 4 * (size_t)6 * ctx().g.vh * sizeof(T) + sizeof(SolverState) + 256;
+}
+
 template <typename T>
 static int invert(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *inv, QPHIX_resid_arg_t *res,
-                  T mass, VectorT<T> *out, VectorT<T> *in) {
+                  T mass, VectorT<T> *out, VectorT<T> *in, char *ws_ext = nullptr) {
   QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called:asqtad_invert", 0);
   QPHIX_REQUIRE(info && L && inv && res && out && in, "asqtad_invert: null argument", 0);
   info->status = QPHIX_FAIL;
@@ -105,7 +112,11 @@ static int invert(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *inv, QPH
   const int vh = ctx().g.vh;
   const size_t nv = 6 * (size_t)vh;
   // workspace: ttt, tmp, p, r, then the device state
-  char *ws = (char *)ws_get(4 * nv * sizeof(T) + sizeof(SolverState) + 256);
+  char *ws = ws_ext ? ws_ext : (char *)ws_get(cg_ws_bytes<T>());
+  if (!ws) {  // out of device memory, error recorded
+    info->status = QPHIX_MEM_ERROR;
+    return 0;
+  }
   T *ttt = (T *)ws, *tmp = ttt + nv, *p = tmp + nv, *r = p + nv;
   SolverState *dst = (SolverState *)(((uintptr_t)(r + nv) + 255) & ~(uintptr_t)255);
   int *gate = &dst->active;
@@ -126,6 +137,9 @@ static int invert(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *inv, QPH
 
   const int batch = steps_per_batch();
   const int nb_dslash = (vh + DSLASH_BLOCK - 1) / DSLASH_BLOCK;
+  // single-reduction steps (FEWSUMS) when asked for and the relative
+  // residue, which needs its own reduction of the updated vectors, is off
+  const bool fs = (fewsums_mode() & 1) && hs.relrsqmin <= 0;
   int total_iters = 0, iteration = 0, nrestart = 0;
   info->final_flop = 0;
   info->status = QPHIX_SUCCESS;
@@ -184,10 +198,16 @@ static int invert(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *inv, QPH
             a2.state = dst;
             a2.scalar_op = SOP_CG_PKP;
             a2.consumer_finalize = true;
-            launch_dslash<T>(*L, parity, ttt, tmp, EPI_PKP, a2);
-            blas_cg_step_c<T>(xd, p, r, ttt, dst, gate, nb_dslash);
+            if (fs) {
+              a2.r = r;
+              launch_dslash<T>(*L, parity, ttt, tmp, EPI_PKP_FS, a2);
+              blas_cg_step_fs<T>(xd, p, r, ttt, dst, gate, nb_dslash);
+            } else {
+              launch_dslash<T>(*L, parity, ttt, tmp, EPI_PKP, a2);
+              blas_cg_step_c<T>(xd, p, r, ttt, dst, gate, nb_dslash);
+            }
           }
-        });
+        }, ws_ext == nullptr);
         state_read(dst, hs);
         if (hs.active == 0) break;
       }
@@ -235,6 +255,10 @@ static int invert_multi(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *in
   const size_t nv = 6 * (size_t)vh;
   char *ws = (char *)ws_get((3 + (size_t)nshift) * nv * sizeof(T) + sizeof(SolverState) +
                             2 * MAXNDMASS * sizeof(T *) + 512);
+  if (!ws) {  // out of device memory, error recorded
+    info->status = QPHIX_MEM_ERROR;
+    return 0;
+  }
   T *ttt = (T *)ws, *tmp = ttt + nv, *r = tmp + nv;
   T *pm0 = r + nv;
   SolverState *dst = (SolverState *)(((uintptr_t)(pm0 + nshift * nv) + 255) & ~(uintptr_t)255);

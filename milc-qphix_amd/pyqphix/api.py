@@ -144,6 +144,21 @@ def gauge_force(G, Fm, coeffs, eps):
     return info
 
 
+def update_gauge(G, Fm, eps):
+    """QPHIX_*3_update_gauge: U <- exp(eps A) U with A the momentum of Fm."""
+    info = Info()
+    G.s.fn(G.prec, "update_gauge")(C.byref(info), G.h, Fm.h, eps)
+    return info
+
+
+def gauge_md(G, Fm, coeffs, eps, nsteps, scheme=0):
+    """QPHIX_*3_gauge_md: pure-gauge MD steps (0 leapfrog, 1 Omelyan)."""
+    info = Info()
+    c = GaugeCoeffs(coeffs[0], coeffs[1], coeffs[2], coeffs[3] if len(coeffs) > 3 else 0.0)
+    G.s.fn(G.prec, "gauge_md")(C.byref(info), G.h, Fm.h, C.byref(c), eps, nsteps, scheme)
+    return info
+
+
 def dslash(L, out, inp, parity):
     L.s.fn(L.prec, "asqtad_dslash")(L.h, out.h, inp.h, parity)
 
@@ -193,13 +208,14 @@ def invert(L, out, inp, mass, parity, resid, relresid=0.0, niter=1000, nrestart=
     return ret, info, res
 
 
-def invert_mixed(L, out, inp, mass, parity, resid, sloppy_bits=32, niter=1000, nrestart=5):
+def invert_mixed(L, out, inp, mass, parity, resid, sloppy_bits=32, niter=1000, nrestart=5,
+                 relresid=0.0):
     """QPHIX_D3_asqtad_invert_mixed: FP32 or FP16-stored sloppy CG with FP64
     reliable updates (D3 links and vectors)."""
     assert L.prec == "D"
     info = Info()
     inv = InvertArg(parity, niter, 0, nrestart, parity)
-    res = ResidArg(resid, 0, 0, 0, 0, 0, 0, 0)
+    res = ResidArg(resid, 0, relresid, 0, 0, 0, 0, 0)
     ret = L.s.lib.QPHIX_D3_asqtad_invert_mixed(C.byref(info), L.h, C.byref(inv), C.byref(res),
                                                mass, out.h, inp.h, sloppy_bits)
     return ret, info, res

@@ -103,6 +103,9 @@ class Lib:
             _sig(lib, f"QPHIX_{P}3_symanzik_1loop_gauge_force", None,
                  [C.POINTER(Info), _dp, _dp, C.POINTER(GaugeCoeffs), R])
             _sig(lib, f"QPHIX_{P}3_asqtad_dslash", None, [_dp, _dp, _dp, C.c_int])
+            _sig(lib, f"QPHIX_{P}3_update_gauge", None, [C.POINTER(Info), _dp, _dp, R])
+            _sig(lib, f"QPHIX_{P}3_gauge_md", None,
+                 [C.POINTER(Info), _dp, _dp, C.POINTER(GaugeCoeffs), R, C.c_int, C.c_int])
             _sig(lib, f"QPHIX_{P}3_asqtad_dslash_time", C.c_double, [_dp, _dp, _dp, C.c_int, C.c_int])
             _sig(lib, f"QPHIX_{P}3_asqtad_invert", C.c_int,
                  [C.POINTER(Info), _dp, C.POINTER(InvertArg), C.POINTER(ResidArg), R, _dp, _dp])
@@ -132,6 +135,8 @@ class Lib:
         _sig(lib, "QPHIX_MI355X_set_sloppy_link_bits", None, [C.c_int])
         _sig(lib, "QPHIX_D3_asqtad_dslash_sloppy", C.c_double, [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int])
         _sig(lib, "QPHIX_MI355X_set_graphs", None, [C.c_int])
+        _sig(lib, "QPHIX_MI355X_debug_fail_alloc", None, [C.c_int])
+        _sig(lib, "QPHIX_MI355X_set_fewsums", None, [C.c_int])
         _sig(lib, "QPHIX_D3_asqtad_links_backcheck", None, [_dp, C.POINTER(C.c_double)])
         _sig(lib, "QPHIX_MI355X_copy_time", C.c_double, [C.c_size_t, C.c_int])
         _sig(lib, "QPHIX_D3_asqtad_invert_mixed", C.c_int,

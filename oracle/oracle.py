@@ -49,11 +49,19 @@ def lib():
             getattr(L, f"oracle_invert_multi_{p}").restype = C.c_int
             getattr(L, f"oracle_invert_multi_{p}").argtypes = [vp, C.c_int, C.c_int, C.c_int,
                                                                C.c_double, vp, C.c_int, vp, vp, vp, vp]
-        L.oracle_relative_residue_d.restype = C.c_double
-        L.oracle_relative_residue_d.argtypes = [vp, vp]
+        for p in ("d", "f"):
+            getattr(L, f"oracle_relative_residue_{p}").restype = C.c_double
+            getattr(L, f"oracle_relative_residue_{p}").argtypes = [vp, vp]
+        for p, R in (("d", C.c_double), ("f", C.c_float)):
+            getattr(L, f"oracle_invert_fewsums_{p}").restype = C.c_int
+            getattr(L, f"oracle_invert_fewsums_{p}").argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                                 R, vp, vp, vp, vp]
         L.oracle_gauge_action.restype = C.c_double
         L.oracle_gauge_action.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int]
         L.oracle_gauge_force.argtypes = [vp, vp, vp, C.c_double, vp, vp]
+        L.oracle_gauge_update.argtypes = [vp, vp, vp, C.c_double]
+        L.oracle_matrix_exp.argtypes = [vp, vp]
+        L.oracle_gauge_md.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int]
         L.oracle_fermion_oprod_d.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
@@ -80,6 +88,33 @@ def gauge_force(dims, U_raw, coef, eps, mom_raw):
     W = np.zeros_like(mom)
     lib().oracle_gauge_force(_p(d), _p(U), _p(c), eps, _p(mom), _p(W))
     return mom, W
+
+
+def gauge_update(dims, U_raw, mom_raw, eps):
+    """U <- exp(eps A) U (raw 72-real/site arrays); returns the new U."""
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    U = np.ascontiguousarray(U_raw, dtype=np.float64).copy()
+    mom = np.ascontiguousarray(mom_raw, dtype=np.float64)
+    lib().oracle_gauge_update(_p(d), _p(U), _p(mom), eps)
+    return U
+
+
+def matrix_exp(X):
+    """exp of one complex 3x3 matrix (the oracle's Taylor / scaling-and-squaring)."""
+    x = np.ascontiguousarray(np.asarray(X, dtype=np.complex128).reshape(3, 3)).view(np.float64).ravel()
+    out = np.empty(18)
+    lib().oracle_matrix_exp(_p(x), _p(out))
+    return out.view(np.complex128).reshape(3, 3)
+
+
+def gauge_md(dims, U_raw, mom_raw, coef, eps, nsteps, scheme=0):
+    """Pure-gauge MD (0 leapfrog, 1 Omelyan); returns (U, mom)."""
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    U = np.ascontiguousarray(U_raw, dtype=np.float64).copy()
+    mom = np.ascontiguousarray(mom_raw, dtype=np.float64).copy()
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    lib().oracle_gauge_md(_p(d), _p(U), _p(mom), _p(c), eps, nsteps, scheme)
+    return U, mom
 
 
 def _p(a):
@@ -125,15 +160,31 @@ class Oracle:
         getattr(self.L, f"oracle_dslash_{self.p}")(self.h, _p(out), _p(inp), cb)
         return out
 
-    def invert(self, b_full, mass, parity, resid, relresid=0.0, niter=1000, nrestart=5, x0=None):
+    def invert(self, b_full, mass, parity, resid, relresid=0.0, niter=1000, nrestart=5, x0=None,
+               fewsums=False):
+        """CG (ks_d_congrad_fn.cpp:221-619); fewsums=True: its FEWSUMS variant
+        (relresid must then be 0)."""
         b = np.ascontiguousarray(b_full, dtype=self.dtype)
         x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=self.dtype).copy()
         od = np.zeros(5)
         oi = np.zeros(2, dtype=np.int32)
-        ret = getattr(self.L, f"oracle_invert_{self.p}")(self.h, parity, niter, nrestart, resid,
-                                                         relresid, mass, _p(x), _p(b), _p(od), _p(oi))
+        if fewsums:
+            assert relresid == 0.0
+            ret = getattr(self.L, f"oracle_invert_fewsums_{self.p}")(self.h, parity, niter, nrestart, resid,
+                                                                     mass, _p(x), _p(b), _p(od), _p(oi))
+        else:
+            ret = getattr(self.L, f"oracle_invert_{self.p}")(self.h, parity, niter, nrestart, resid,
+                                                             relresid, mass, _p(x), _p(b), _p(od), _p(oi))
         return x, dict(ret=ret, final_rsq=od[0], final_rel=od[1], size_r=od[2], size_relr=od[3],
                        final_flop=od[4], final_iter=int(oi[0]), final_restart=int(oi[1]))
+
+    def relative_residue(self, r_half, x_half):
+        """sqrt(2 sum_x |r(x)|^2 / |x(x)|^2 / V) over one parity's sites
+        (ks_blas_utils_c.cpp:110-149), in this oracle's precision."""
+        r = np.ascontiguousarray(r_half, dtype=self.dtype)
+        x = np.ascontiguousarray(x_half, dtype=self.dtype)
+        assert r.size == x.size == 6 * self.vh
+        return getattr(self.L, f"oracle_relative_residue_{self.p}")(_p(r), _p(x))
 
     def invert_multi(self, b_full, masses, parity, resid, niter=1000, nrestart=5):
         b = np.ascontiguousarray(b_full, dtype=self.dtype)

@@ -170,14 +170,18 @@ template <typename T, typename F> double blocked_sum(long n, F term) {
   return (double)tot;
 }
 
-/* ks_blas_utils_c.cpp:110-149 */
+/* ks_blas_utils_c.cpp:110-149.  Per site and colour the reference squares and
+ * adds the real and imaginary parts in the vector type (fptype) and widens the
+ * result to double (calc_soalen_rel_residue, :118-121); sites with r = 0 are
+ * skipped, sites with x = 0 count 1. */
 template <typename T> double relative_residue(const T *r, const T *x) {
   double res = 0;
   for (long i = 0; i < G.vh; ++i) {
     double num = 0, den = 0;
-    for (int e = 0; e < 6; ++e) {
-      num += (double)r[6 * i + e] * r[6 * i + e];
-      den += (double)x[6 * i + e] * x[6 * i + e];
+    for (int c = 0; c < 3; ++c) {
+      const T *rc = r + 6 * i + 2 * c, *xc = x + 6 * i + 2 * c;
+      num += (double)(T)(rc[0] * rc[0] + rc[1] * rc[1]);
+      den += (double)(T)(xc[0] * xc[0] + xc[1] * xc[1]);
     }
     if (num == 0) continue;
     res += den == 0 ? 1.0 : num / den;
@@ -191,17 +195,24 @@ struct InvertOut {
 };
 
 /* ks_d_congrad_fn.cpp:221-619 restated.  `x` and `b` are full raw arrays
- * (even block then odd block); parity is the API parity. */
+ * (even block then odd block); parity is the API parity.
+ *
+ * fewsums = 1 restates the FEWSUMS variant (ks_d_congrad_fn.cpp:260-263, 402,
+ * 450-541) with the sums the C++ port leaves out taken from MILC's code kept
+ * in its comments (:119-128, :150-158): the p.Ap pass also sums
+ * c_tr = Re<ttt|r> and c_tt = <ttt|ttt>; the x/r pass sums the actual |r|^2,
+ * used as oldrsq one step later; rsq = oldrsq + 2 a c_tr + a^2 c_tt replaces
+ * the summed |r|^2 for the stopping test and b. */
 template <typename T>
 void invert(const Links<T> *L, int parity_api, int niter, int max_restarts, double resid,
-            double relresid, T mass, T *x_full, const T *b_full, InvertOut *o) {
+            double relresid, T mass, T *x_full, const T *b_full, InvertOut *o, int fewsums = 0) {
   const long n = 6 * G.vh;
   std::vector<T> ttt(n), tmp(n), p(n), r(n);
   const T rsqmin = (T)resid, relrsqmin = (T)relresid;
   const int max_cg = max_restarts * niter;
   const T msq_x4 = (T)(4.0 * mass * mass);
   int total_iters = 0, iteration = 0, nrestart = 0;
-  double rsq = 0, relrsq = 0, oldrsq, pkp;
+  double rsq = 0, relrsq = 0, oldrsq, pkp, actual_rsq = 0, c_tr = 0, c_tt = 0;
   double size_r = 0, size_relr = 0, final_rsq = 0, final_rel = 0;
   int todo[2], ntodo = 0;
   if (parity_api == P_EVENODD) { todo[0] = 0; todo[1] = 1; ntodo = 2; }
@@ -235,6 +246,7 @@ void invert(const Links<T> *L, int parity_api, int niter, int max_restarts, doub
           return r[i] * r[i];
         });
         if (relrsqmin > 0) relrsq = relative_residue(r.data(), xd);
+        actual_rsq = rsq;
         final_rsq = (T)rsq / source_norm;
         final_rel = (T)relrsq;
         iteration++;
@@ -243,7 +255,7 @@ void invert(const Links<T> *L, int parity_api, int niter, int max_restarts, doub
           break;
         if (iteration != 0) nrestart++;
       }
-      oldrsq = rsq;
+      oldrsq = fewsums ? actual_rsq : rsq;
       dslash(L, tmp.data(), p.data(), other);
       dslash(L, ttt.data(), tmp.data(), parity);
       /* calc_pkp, ks_d_congrad_fn.cpp:159-171 */
@@ -251,14 +263,24 @@ void invert(const Links<T> *L, int parity_api, int niter, int max_restarts, doub
         ttt[i] = ttt[i] + (-msq_x4) * p[i];
         return p[i] * ttt[i];
       });
+      if (fewsums) {
+        c_tr = blocked_sum<T>(n, [&](long i) { return ttt[i] * r[i]; });
+        c_tt = blocked_sum<T>(n, [&](long i) { return ttt[i] * ttt[i]; });
+      }
       iteration++;
       const T a = (T)(-rsq / pkp);
       /* calc_rsq2, ks_d_congrad_fn.cpp:131-147 */
-      rsq = blocked_sum<T>(n, [&](long i) {
+      const double summed = blocked_sum<T>(n, [&](long i) {
         xd[i] = xd[i] + a * p[i];
         r[i] = r[i] + a * ttt[i];
         return r[i] * r[i];
       });
+      if (fewsums) {
+        actual_rsq = summed;
+        rsq = oldrsq + 2.0 * a * c_tr + (double)a * a * c_tt;  // :531
+      } else {
+        rsq = summed;
+      }
       if (relrsqmin > 0) relrsq = relative_residue(r.data(), xd);
       size_r = (T)rsq / source_norm;
       size_relr = (T)relrsq;
@@ -684,6 +706,100 @@ void oracle_gauge_force(const int *latsize, const double *U, const double *coef,
 }
 
 
+/* U <- exp(eps A) U for every link (update_mg_body's gauge update with
+ * epsilonU = eps, gauge_force_imp.cpp:185,334; MILC's update_u).  U and mom are
+ * raw 72-real/site arrays (mom anti-Hermitian).  The exponential is the plain
+ * matrix Taylor series with scaling and squaring (Moler & Van Loan 2003): the
+ * argument is halved until its max-row-sum norm is below 1/2, 30 terms are
+ * summed and the result squared back. */
+static M3 mexp(M3 X) {
+  double nrm = 0;
+  for (int i = 0; i < 3; ++i) {
+    double r = 0;
+    for (int j = 0; j < 3; ++j) r += std::abs(X.a[3 * i + j]);
+    nrm = r > nrm ? r : nrm;
+  }
+  int s = 0;
+  while (nrm > 0.5) {
+    nrm *= 0.5;
+    ++s;
+  }
+  const double sc = std::ldexp(1.0, -s);
+  for (int e = 0; e < 9; ++e) X.a[e] *= sc;
+  M3 E = meye(), term = meye();
+  for (int n = 1; n <= 30; ++n) {
+    term = mmul(term, X);
+    for (int e = 0; e < 9; ++e) term.a[e] /= (double)n;
+    for (int e = 0; e < 9; ++e) E.a[e] += term.a[e];
+  }
+  for (int k = 0; k < s; ++k) E = mmul(E, E);
+  return E;
+}
+
+static void gauge_update(const GLat &g, double *U, const double *mom, double eps) {
+#pragma omp parallel for schedule(static)
+  for (long l = 0; l < 8 * g.vh; ++l) {
+    M3 A, u;
+    for (int e = 0; e < 9; ++e) {
+      A.a[e] = eps * cd(mom[18 * l + 2 * e], mom[18 * l + 2 * e + 1]);
+      u.a[e] = cd(U[18 * l + 2 * e], U[18 * l + 2 * e + 1]);
+    }
+    const M3 r = mmul(mexp(A), u);
+    for (int e = 0; e < 9; ++e) {
+      U[18 * l + 2 * e] = r.a[e].real();
+      U[18 * l + 2 * e + 1] = r.a[e].imag();
+    }
+  }
+}
+
+static GLat glat(const int *latsize) {
+  GLat g;
+  for (int d = 0; d < 4; ++d) g.L[d] = latsize[d];
+  g.vh = (long)latsize[0] * latsize[1] * latsize[2] * latsize[3] / 2;
+  return g;
+}
+
+void oracle_gauge_update(const int *latsize, double *U, const double *mom, double eps) {
+  gauge_update(glat(latsize), U, mom, eps);
+}
+
+void oracle_matrix_exp(const double *X, double *out) {
+  M3 m;
+  for (int e = 0; e < 9; ++e) m.a[e] = cd(X[2 * e], X[2 * e + 1]);
+  const M3 r = mexp(m);
+  for (int e = 0; e < 9; ++e) {
+    out[2 * e] = r.a[e].real();
+    out[2 * e + 1] = r.a[e].imag();
+  }
+}
+
+/* Pure-gauge molecular dynamics, nsteps of size eps: scheme 0 leapfrog (force
+ * half step, then gauge and force steps, force half step at the end), scheme 1
+ * Omelyan's 2MN (lambda = 0.1931833275037836; Omelyan, Mryglod & Folk 2003),
+ * adjacent force steps merged.  Force steps mom <- mom - e TA(W) as
+ * oracle_gauge_force, gauge steps U <- exp(e A) U. */
+void oracle_gauge_md(const int *latsize, double *U, double *mom, const double *coef, double eps, int nsteps,
+                     int scheme) {
+  const GLat g = glat(latsize);
+  auto force = [&](double e) { oracle_gauge_force(latsize, U, coef, e, mom, nullptr); };
+  if (scheme == 0) {
+    for (int k = 0; k < nsteps; ++k) {
+      force(k == 0 ? 0.5 * eps : eps);
+      gauge_update(g, U, mom, eps);
+    }
+    if (nsteps > 0) force(0.5 * eps);
+  } else {
+    const double lam = 0.1931833275037836;
+    for (int k = 0; k < nsteps; ++k) {
+      force(k == 0 ? lam * eps : 2 * lam * eps);
+      gauge_update(g, U, mom, 0.5 * eps);
+      force((1 - 2 * lam) * eps);
+      gauge_update(g, U, mom, 0.5 * eps);
+    }
+    if (nsteps > 0) force(lam * eps);
+  }
+}
+
 /* Lattice set-up; the reference's minimum-size checks qphix_misc.cpp:494-497. */
 int oracle_setup(const int *latsize) {
   if (latsize[0] % 4 || latsize[1] % 4 || latsize[2] % 4 || latsize[3] % 4) return 1;
@@ -725,6 +841,26 @@ void oracle_dslash_f(void *L, float *out, const float *in, int cb) {
 }
 
 double oracle_relative_residue_d(const double *r, const double *x) { return relative_residue(r, x); }
+
+int oracle_invert_fewsums_d(void *L, int parity, int niter, int nrestart, double resid, double mass, double *x,
+                            const double *b, double *out_d, int *out_i) {
+  InvertOut o{};
+  invert((Links<double> *)L, parity, niter, nrestart, resid, 0.0, mass, x, b, &o, 1);
+  out_d[0] = o.final_rsq; out_d[1] = o.final_rel; out_d[2] = o.size_r; out_d[3] = o.size_relr;
+  out_d[4] = o.final_flop;
+  out_i[0] = o.final_iter; out_i[1] = o.final_restart;
+  return o.ret;
+}
+int oracle_invert_fewsums_f(void *L, int parity, int niter, int nrestart, double resid, float mass, float *x,
+                            const float *b, double *out_d, int *out_i) {
+  InvertOut o{};
+  invert((Links<float> *)L, parity, niter, nrestart, resid, 0.0, mass, x, b, &o, 1);
+  out_d[0] = o.final_rsq; out_d[1] = o.final_rel; out_d[2] = o.size_r; out_d[3] = o.size_relr;
+  out_d[4] = o.final_flop;
+  out_i[0] = o.final_iter; out_i[1] = o.final_restart;
+  return o.ret;
+}
+double oracle_relative_residue_f(const float *r, const float *x) { return relative_residue(r, x); }
 
 int oracle_invert_d(void *L, int parity, int niter, int nrestart, double resid, double relresid,
                     double mass, double *x, const double *b, double *out_d, int *out_i) {

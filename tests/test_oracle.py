@@ -241,19 +241,164 @@ def test_multishift_plane_wave():
         assert rel_l2(half(x, lat, 1), half(b, lat, 1) / lam) < 1e-12
 
 
-def test_relative_residue_definition():
-    """sqrt(2 sum_x |r(x)|^2/|x(x)|^2 / V), sites with r=0 skipped, x=0 counts 1
-    (ks_blas_utils_c.cpp:110-149)."""
-    lat = F.Lattice((4, 4, 4, 4))
-    make_oracle(lat, random_links(lat, seed=1))
+def _rr(orc, r, x):
+    return orc.relative_residue(r, x)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_relative_residue_closed_forms(dtype):
+    """relative_residue = sqrt(2 sum_x |r(x)|^2/|x(x)|^2 / V) over one parity
+    (ks_blas_utils_c.cpp:110-149), pinned by values fixed without its formula:
+    r = c x at every site gives |c|; r = 0 gives 0; x = 0 with r != 0 at every
+    site gives sqrt(2 V_h / V) = 1; a mixture of the three kinds of site gives
+    the hand-computed 1.5."""
+    lat = F.Lattice((4, 4, 4, 8))
+    orc = make_oracle(lat, random_links(lat, seed=1), dtype=dtype)
+    tol = 1e-13 if dtype == np.float64 else 2e-6
     rng = np.random.default_rng(3)
-    r = rng.standard_normal(6 * lat.vh)
-    x = rng.standard_normal(6 * lat.vh)
-    r[:6] = 0
-    x[6:12] = 0
-    num = (r.reshape(-1, 6) ** 2).sum(1)
-    den = (x.reshape(-1, 6) ** 2).sum(1)
-    terms = np.where(num == 0, 0.0, np.where(den == 0, 1.0, num / np.where(den == 0, 1, den)))
-    exp = np.sqrt(2 * terms.sum() / lat.V)
-    got = O.lib().oracle_relative_residue_d(O._p(r), O._p(x))
-    assert abs(got - exp) < 1e-14 * exp
+    x = rng.standard_normal(6 * lat.vh).astype(dtype)
+    for c in (0.5, 3.0, 1e-4, 2.0 ** -20):
+        got = _rr(orc, (c * x.astype(np.float64)).astype(dtype), x)
+        assert abs(got - c) < tol * c, (c, got)
+    # a complex factor c acts per colour as a rotation: |c r| = |c||r|
+    xc = x.astype(np.float64).view(np.complex128)
+    rc = ((0.6 - 0.8j) * 2.5 * xc).view(np.float64).astype(dtype)
+    assert abs(_rr(orc, rc, x) - 2.5) < tol * 2.5
+    assert _rr(orc, np.zeros_like(x), x) == 0.0
+    assert abs(_rr(orc, x, np.zeros_like(x)) - 1.0) < tol
+    # sites: first half r = 2x (term 4), next quarter r = 0 (skipped),
+    # last quarter x = 0 (term 1): sqrt(2 (vh/2 * 4 + vh/4) / (2 vh)) = 1.5
+    r = (2 * x.astype(np.float64)).astype(dtype).reshape(lat.vh, 6)
+    xm = x.copy().reshape(lat.vh, 6)
+    h, q = lat.vh // 2, lat.vh // 4
+    r[h:h + q] = 0
+    xm[h + q:] = 0
+    assert abs(_rr(orc, r.ravel(), xm.ravel()) - 1.5) < tol * 1.5
+    # a planted swap of r and x changes the mixed case (x = 0 sites would be
+    # skipped, r = 0 sites count 1): the function is not symmetric
+    assert abs(_rr(orc, xm.ravel(), r.ravel()) - 1.5) > 0.1
+
+
+@pytest.mark.parametrize("parity", [O.EVEN, O.ODD])
+def test_cg_relative_residue_plane_wave(parity):
+    """CG with the relative-residue stop alone (resid = 0) on the free field,
+    where everything is known in closed form: with initial guess x0 = chi and
+    source b = (lambda + kappa) chi for a plane wave chi of eigenvalue lambda,
+    the first true residual is r = kappa chi = kappa x0, so final_rel = |kappa|
+    exactly.  relresid above |kappa| stops there (1 iteration by the
+    reference's counting); below it, CG converges in one step (3 iterations)
+    to x = b / lambda with a relative residue at rounding level."""
+    lat = F.Lattice((8, 4, 4, 8))
+    c1, c3 = 0.9, -0.07
+    fat, lng = F.free_links(lat, c1, c3)
+    orc = make_oracle(lat, (fat, lng, F.back_links(lat, fat, 1), F.back_links(lat, lng, 3)))
+    cb = 0 if parity == O.EVEN else 1
+    p = allowed_momenta(lat, 3)[2]
+    m = 0.1
+    lam = F.free_eigenvalue(p, c1, c3, 4 * m * m)
+    chi = F.to_raw(F.plane_wave(lat, p, 1, parity=cb))
+    for kappa in (0.03, -0.2):
+        b = (lam + kappa) * chi
+        x, info = orc.invert(b, m, parity, resid=0.0, relresid=0.5, x0=chi)
+        assert info["final_iter"] == 1 and info["ret"] == 1
+        assert abs(info["final_rel"] - abs(kappa)) < 1e-12 * abs(kappa)
+        assert rel_l2(x, chi) == 0.0
+        x, info = orc.invert(b, m, parity, resid=0.0, relresid=1e-3, x0=chi)
+        assert info["final_iter"] == 3
+        assert info["final_rel"] < 1e-12 and info["size_relr"] < 1e-12
+        assert rel_l2(half(x, lat, cb), half(b, lat, cb) / lam) < 1e-13
+    # with both stops, the absolute one can hold the solver after the relative
+    # one is met: resid 1e-40 is never met in one step, relresid 0.5 is
+    x, info = orc.invert((lam + 0.03) * chi, m, parity, resid=1e-40, relresid=0.5, x0=chi, niter=4,
+                         nrestart=1)
+    assert info["final_iter"] > 3
+
+
+@pytest.mark.parametrize("parity", [O.EVEN, O.ODD])
+def test_cg_float_matches_dense_solve(parity):
+    """The F build's CG (scalars rounded to float, reductions in float as the
+    reference's fptype accumulators) against a dense FP64 solve at 4^4."""
+    lat = F.Lattice((4, 4, 4, 4))
+    links = random_links(lat, "weak", seed=33)
+    orc = make_oracle(lat, links, dtype=np.float32)
+    orc64 = make_oracle(lat, links)
+    cb = 0 if parity == O.EVEN else 1
+    m = 0.1
+    b = F.to_raw(F.gaussian_source(lat, 34, parity=cb))
+    x, info = orc.invert(b.astype(np.float32), m, parity, resid=1e-10)
+    A = 4 * m * m * np.eye(3 * lat.vh) - dense_ddag(orc64, lat, cb)
+    xs = np.linalg.solve(A, half(b, lat, cb).view(np.complex128))
+    assert x.dtype == np.float32
+    assert rel_l2(half(x, lat, cb).astype(np.float64).view(np.complex128), xs) < 1e-5
+    assert info["final_rsq"] < 1e-10
+
+
+def test_multishift_float_matches_dense_solves():
+    lat = F.Lattice((4, 4, 4, 4))
+    links = random_links(lat, "weak", seed=53)
+    orc = make_oracle(lat, links, dtype=np.float32)
+    orc64 = make_oracle(lat, links)
+    masses = np.array([0.2, 0.05, 0.1, 0.4])
+    b = F.to_raw(F.gaussian_source(lat, 54, parity=0))
+    xs, info = orc.invert_multi(b.astype(np.float32), masses.astype(np.float32), O.EVEN, resid=1e-10)
+    DD = dense_ddag(orc64, lat, 0)
+    bh = half(b, lat, 0).view(np.complex128)
+    for m, x in zip(masses, xs):
+        ref = np.linalg.solve(4 * m * m * np.eye(3 * lat.vh) - DD, bh)
+        assert rel_l2(half(x, lat, 0).astype(np.float64).view(np.complex128), ref) < 2e-5
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("parity", [O.EVEN, O.ODD])
+def test_cg_fewsums_matches_dense_solve(dtype, parity):
+    """The FEWSUMS variant of the CG (single reduction per step, |r|^2 from
+    the recurrence |r|^2 + 2a Re<Ap|r> + a^2 |Ap|^2) solves the same system:
+    against a dense FP64 solve at 4^4, with restarts every 25 iterations."""
+    lat = F.Lattice((4, 4, 4, 4))
+    links = random_links(lat, "weak", seed=35)
+    orc = make_oracle(lat, links, dtype=dtype)
+    orc64 = orc if dtype == np.float64 else make_oracle(lat, links)
+    cb = 0 if parity == O.EVEN else 1
+    m = 0.1
+    b = F.to_raw(F.gaussian_source(lat, 36, parity=cb))
+    resid = 1e-24 if dtype == np.float64 else 1e-10
+    A = 4 * m * m * np.eye(3 * lat.vh) - dense_ddag(orc64, lat, cb)
+    xs = np.linalg.solve(A, half(b, lat, cb).view(np.complex128))
+    for niter in (1000, 25):
+        x, info = orc.invert(b.astype(dtype), m, parity, resid=resid, niter=niter, nrestart=50,
+                             fewsums=True)
+        # FP32: the stop is tested on the recurrence |r|^2, ~1e-10 from the true
+        # one, so the error is a few times the plain CG's 1e-5
+        tol = 1e-10 if dtype == np.float64 else 5e-5
+        assert rel_l2(half(x, lat, cb).astype(np.float64).view(np.complex128), xs) < tol
+        assert info["final_rsq"] < resid
+
+
+def test_cg_fewsums_plane_wave_and_counts():
+    """Free-field plane wave: one step, 3 iterations, x = b / lambda; on the
+    8^4 random-link system the iteration count stays within 5% of the plain
+    CG's, and the final (true, restart-evaluated) residual meets the target.
+    The recurrence |r|^2 + 2a Re<Ap|r> + a^2 |Ap|^2 cancels to rounding level,
+    ~1e-16 |r|^2 in one step, so the one-step known answer takes a target
+    above that (1e-12); with 1e-24 the recurrence cannot register the exact
+    drop and the solve runs to its restart interval, where the true residual
+    ends it."""
+    lat = F.Lattice((8, 4, 4, 8))
+    c1, c3 = 0.9, -0.07
+    fat, lng = F.free_links(lat, c1, c3)
+    orc = make_oracle(lat, (fat, lng, F.back_links(lat, fat, 1), F.back_links(lat, lng, 3)))
+    p = allowed_momenta(lat, 2)[0]
+    b = F.to_raw(F.plane_wave(lat, p, 2, parity=0))
+    x, info = orc.invert(b, 0.1, O.EVEN, resid=1e-12, fewsums=True)
+    assert info["final_iter"] == 3
+    _, info24 = orc.invert(b, 0.1, O.EVEN, resid=1e-24, fewsums=True, niter=50)
+    assert info24["final_iter"] > 51 and info24["final_rsq"] < 1e-24
+    lam = F.free_eigenvalue(p, c1, c3, 0.04)
+    assert rel_l2(half(x, lat, 0), half(b, lat, 0) / lam) < 1e-12
+    lat = F.Lattice((8, 8, 8, 8))
+    orc = make_oracle(lat, random_links(lat, "haar", 1001))
+    b = F.to_raw(F.gaussian_source(lat, 77, parity=0))
+    _, i0 = orc.invert(b, 0.025, O.EVEN, resid=1e-16)
+    x, i1 = orc.invert(b, 0.025, O.EVEN, resid=1e-16, fewsums=True)
+    assert abs(i1["final_iter"] - i0["final_iter"]) <= 0.05 * i0["final_iter"]
+    assert i1["final_rsq"] < 1e-16
