@@ -122,6 +122,51 @@ int QPHIX_D3_asqtad_invert_mixed(QPHIX_info_t *info, QPHIX_D3_FermionLinksAsqtad
                                  double mass, QPHIX_D3_ColorVector *out, QPHIX_D3_ColorVector *in,
                                  int sloppy_bits);
 
+/* ---- domain decomposition on one GPU ------------------------------------------
+ * asqtad_dslash of a lattice cut into domains along the directions of
+ * cut_mask (bit mu, extent >= 8 along each), with every domain exchanging its
+ * 3-deep faces with itself (self-loopback, the reference's ASSUME_MULTINODE
+ * build, ks_boundary.cpp:199,250-256): the faces of `in` are packed
+ * (ks_boundary.cpp:440-556), the interior sites computed from local
+ * neighbours, and the boundary sites from local neighbours plus the received
+ * faces (:558-690).  The result is bit-identical to asqtad_dslash.  reps > 0
+ * returns the average milliseconds of reps calls; reps = 0 makes one call
+ * and returns 0. */
+double QPHIX_D3_asqtad_dslash_faces(QPHIX_D3_FermionLinksAsqtad *asqtad, QPHIX_D3_ColorVector *out,
+                                    QPHIX_D3_ColorVector *in, QPHIX_evenodd_t parity, int cut_mask, int reps);
+double QPHIX_F3_asqtad_dslash_faces(QPHIX_F3_FermionLinksAsqtad *asqtad, QPHIX_F3_ColorVector *out,
+                                    QPHIX_F3_ColorVector *in, QPHIX_evenodd_t parity, int cut_mask, int reps);
+
+/* ---- fermion links from the gauge field ---------------------------------------
+ * Asqtad coefficients per path, in the physical sign convention (links
+ * without staggered phases): fat link = one_link U + three_staple (each of 6
+ * staples) + five_staple (each of 24 ordered five-link staples) + seven_staple
+ * (each of 48) + lepage (each of 6 two-step staples); naik multiplies
+ * U U U.  Asqtad: {5/8, -1/24, 1/16, 1/64, 1/384, -1/16}; fat7 alone:
+ * {1/8, 0, 1/16, 1/64, 1/384, 0}.  Tadpole factors go into the coefficients. */
+typedef struct {
+  double one_link, naik, three_staple, five_staple, seven_staple, lepage;
+} QPHIX_MI355X_asqtad_coeffs_t;
+/* Fat and Naik links of the thin SU(3) field G (no phases), built on the GPU
+ * with MILC's staggered phases and (flags bit 0 clear) the antiperiodic time
+ * boundary folded in, and MILC's double-store back links: the handle equals
+ * asqtad_create_L_from_raw of those links (MILC forms them on the host and
+ * the reference's own rephase/long-link kernels are disabled,
+ * ks_dslash_complete_specialization.cpp:238-275).  Arithmetic in FP64. */
+QPHIX_D3_FermionLinksAsqtad *QPHIX_D3_asqtad_create_L_from_G(QPHIX_info_t *info,
+                                                             const QPHIX_MI355X_asqtad_coeffs_t *coeffs,
+                                                             QPHIX_D3_GaugeField *G, int flags);
+QPHIX_F3_FermionLinksAsqtad *QPHIX_F3_asqtad_create_L_from_G(QPHIX_info_t *info,
+                                                             const QPHIX_MI355X_asqtad_coeffs_t *coeffs,
+                                                             QPHIX_F3_GaugeField *G, int flags);
+/* HISQ links (Follana et al., Phys. Rev. D 75 (2007) 054502): fat7, projection
+ * to U(3), then the second-level smearing {1, -(1 + naik_eps)/24, 1/16, 1/64,
+ * 1/384, -1/8} of the projected field; phases and boundary as above. */
+QPHIX_D3_FermionLinksAsqtad *QPHIX_D3_hisq_create_L_from_G(QPHIX_info_t *info, QPHIX_D3_GaugeField *G,
+                                                           double naik_eps, int flags);
+QPHIX_F3_FermionLinksAsqtad *QPHIX_F3_hisq_create_L_from_G(QPHIX_info_t *info, QPHIX_F3_GaugeField *G,
+                                                           double naik_eps, int flags);
+
 /* ---- gauge-field update and molecular dynamics --------------------------------
  * U_mu(x) <- exp(eps A_mu(x)) U_mu(x) for every link of G (both parities), A
  * the traceless anti-Hermitian momentum of F (H = -iA, MILC's 8-real layout).

@@ -73,16 +73,11 @@ __device__ __forceinline__ int block_slot(int opt) {
 // Applies one direction's link pair (forward link on in(fwd), back link on
 // in(bwd)).  NC = 18: both links complete; NC = 12: rows 0-1 of each, row 2
 // rebuilt as sign * conj(r0 x r1).
-template <typename TL, int NC, typename SI, bool NT>
-__device__ __forceinline__ void apply_pair(const typename vec16<TL>::type *base, int vh, int i,
-                                           const SI *in, int nf, int nb, unsigned sg_f,
-                                           unsigned sg_b, cplx<typename link_compute<TL>::type> *acc) {
-  using TC = typename link_compute<TL>::type;
-  cplx<TC> u[NC];
-  PairLoad<TL, NC, NT>::load(base, vh, i, u);
-  cplx<TC> vf[3], vb[3];
-  Spin<SI>::load(in, nf, vf);
-  Spin<SI>::load(in, nb, vb);
+// The arithmetic of one pair: acc += U_f vf + U_b vb, u the pair as loaded.
+// Shared by every kernel that must give the same sums bit for bit.
+template <int NC, typename TC>
+__device__ __forceinline__ void pair_compute(const cplx<TC> *u, const cplx<TC> *vf, const cplx<TC> *vb,
+                                             unsigned sg_f, unsigned sg_b, cplx<TC> *acc) {
   if constexpr (NC == 18) {
     matvec_acc(u, vf, acc);
     matvec_acc(u + 9, vb, acc);
@@ -97,6 +92,19 @@ __device__ __forceinline__ void apply_pair(const typename vec16<TL>::type *base,
     recon_row2(w, sg_b ? (TC)-1 : (TC)1);
     matvec_acc(w, vb, acc);
   }
+}
+
+template <typename TL, int NC, typename SI, bool NT>
+__device__ __forceinline__ void apply_pair(const typename vec16<TL>::type *base, int vh, int i,
+                                           const SI *in, int nf, int nb, unsigned sg_f,
+                                           unsigned sg_b, cplx<typename link_compute<TL>::type> *acc) {
+  using TC = typename link_compute<TL>::type;
+  cplx<TC> u[NC];
+  PairLoad<TL, NC, NT>::load(base, vh, i, u);
+  cplx<TC> vf[3], vb[3];
+  Spin<SI>::load(in, nf, vf);
+  Spin<SI>::load(in, nb, vb);
+  pair_compute<NC, TC>(u, vf, vb, sg_f, sg_b, acc);
 }
 
 // Occupancy target (waves per SIMD) for k_dslash, for measurements; unset, the

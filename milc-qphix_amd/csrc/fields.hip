@@ -455,11 +455,14 @@ template <typename T> static void destroy_V(typename Handles<T>::V *V) {
 }
 
 // ------------------------------------------------------------------ L
-template <typename T>
-static typename Handles<T>::L *create_L(const T *fat, const T *lng, const T *fatback,
-                                        const T *lngback, int evenodd) {
-  QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called:create_L", nullptr);
-  QPHIX_REQUIRE(fat && lng && fatback && lngback, "create_L: null link array", nullptr);
+// The import pipeline of create_L: fill(cb, stage, long_only) puts parity
+// cb's raw links in MILC order into the device staging buffer, fat at stage,
+// fat back at + n, long at + 2n, long back at + 3n (n = 72 vh reals); with
+// long_only only the last two are needed.  Pass 1 decides the Naik storage
+// (12 reals only if every supplied long link is SU(3) times a sign), pass 2
+// reorders into the tile-major pairs.  Returns false when the fill failed.
+template <typename T, typename Fill>
+static typename Handles<T>::L *create_L_core(int evenodd, Fill &&fill) {
   const int vh = ctx().g.vh;
   const size_t nraw = 72 * (size_t)vh;
   auto *L = new typename Handles<T>::L;
@@ -468,25 +471,26 @@ static typename Handles<T>::L *create_L(const T *fat, const T *lng, const T *fat
   unsigned long long *dmax = (unsigned long long *)dev_alloc(sizeof(unsigned long long) * 4);
   if (!stage || !dmax) { dev_free(stage); dev_free(dmax); delete L; return nullptr; }
   uint8_t *signs[2] = {nullptr, nullptr};
+  auto bail = [&]() -> typename Handles<T>::L * {
+    sync();
+    dev_free(signs[0]);
+    dev_free(signs[1]);
+    dev_free(stage);
+    dev_free(dmax);
+    links_free<T>(L);
+    delete L;
+    return nullptr;
+  };
   // Pass 1: decide the long-link storage from every supplied long link.
   bool can12 = long_recon_mode() != 18;
   for (int cb = 0; cb < 2 && can12; ++cb) {
     if (!(evenodd & (cb ? QPHIX_ODD : QPHIX_EVEN))) continue;
-    const size_t off = cb ? nraw : 0;
-    h2d(stage, lng + off, sizeof(T) * nraw);
-    h2d(stage + nraw, lngback + off, sizeof(T) * nraw);
+    if (!fill(cb, stage, true)) return bail();
     HIP_CHECK(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), ctx().stream));
     signs[cb] = (uint8_t *)dev_alloc((size_t)vh);
-    if (!signs[cb]) {  // error recorded
-      dev_free(signs[0]);
-      dev_free(signs[1]);
-      dev_free(stage);
-      dev_free(dmax);
-      delete L;
-      return nullptr;
-    }
-    hipLaunchKernelGGL(k_recon_check<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, stage,
-                       stage + nraw, vh, signs[cb], dmax);
+    if (!signs[cb]) return bail();  // error recorded
+    hipLaunchKernelGGL(k_recon_check<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, stage + 2 * nraw,
+                       stage + 3 * nraw, vh, signs[cb], dmax);
     unsigned long long bits = 0;
     d2h(&bits, dmax, sizeof(bits));
     sync();
@@ -498,23 +502,10 @@ static typename Handles<T>::L *create_L(const T *fat, const T *lng, const T *fat
   const int ncl = can12 ? 6 : 9;
   for (int cb = 0; cb < 2; ++cb) {
     if (!(evenodd & (cb ? QPHIX_ODD : QPHIX_EVEN))) continue;
-    const size_t off = cb ? nraw : 0;
-    h2d(stage, fat + off, sizeof(T) * nraw);
-    h2d(stage + nraw, fatback + off, sizeof(T) * nraw);
-    h2d(stage + 2 * nraw, lng + off, sizeof(T) * nraw);
-    h2d(stage + 3 * nraw, lngback + off, sizeof(T) * nraw);
+    if (!fill(cb, stage, false)) return bail();
     L->fat[cb] = (T *)dev_alloc(sizeof(T) * 144 * link_sites(vh));
     L->lng[cb] = (T *)dev_alloc(sizeof(T) * 2 * 2 * ncl * 4 * link_sites(vh));
-    if (!L->fat[cb] || !L->lng[cb]) {  // error recorded
-      sync();
-      dev_free(signs[0]);
-      dev_free(signs[1]);
-      dev_free(stage);
-      dev_free(dmax);
-      links_free<T>(L);
-      delete L;
-      return nullptr;
-    }
+    if (!L->fat[cb] || !L->lng[cb]) return bail();  // error recorded
     hipLaunchKernelGGL(k_import_pairs<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, L->fat[cb],
                        stage, stage + nraw, vh, 9);
     hipLaunchKernelGGL(k_import_pairs<T>, dim3(nblk(vh)), dim3(256), 0, ctx().stream, L->lng[cb],
@@ -529,6 +520,32 @@ static typename Handles<T>::L *create_L(const T *fat, const T *lng, const T *fat
   dev_free(dmax);
   return L;
 }
+
+template <typename T>
+static typename Handles<T>::L *create_L(const T *fat, const T *lng, const T *fatback,
+                                        const T *lngback, int evenodd) {
+  QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called:create_L", nullptr);
+  QPHIX_REQUIRE(fat && lng && fatback && lngback, "create_L: null link array", nullptr);
+  const size_t nraw = 72 * (size_t)ctx().g.vh;
+  return create_L_core<T>(evenodd, [&](int cb, T *stage, bool long_only) {
+    const size_t off = cb ? nraw : 0;
+    if (!long_only) {
+      h2d(stage, fat + off, sizeof(T) * nraw);
+      h2d(stage + nraw, fatback + off, sizeof(T) * nraw);
+    }
+    h2d(stage + 2 * nraw, lng + off, sizeof(T) * nraw);
+    h2d(stage + 3 * nraw, lngback + off, sizeof(T) * nraw);
+    return true;
+  });
+}
+
+// Links whose raw MILC-order blocks a device kernel writes (link_prep.hip).
+template <typename T>
+typename Handles<T>::L *create_L_device(int evenodd, const std::function<bool(int, T *)> &fill) {
+  return create_L_core<T>(evenodd, [&](int cb, T *stage, bool) { return fill(cb, stage); });
+}
+template Handles<double>::L *create_L_device<double>(int, const std::function<bool(int, double *)> &);
+template Handles<float>::L *create_L_device<float>(int, const std::function<bool(int, float *)> &);
 
 template <typename T>
 static void extract_L(T *fat, T *lng, const typename Handles<T>::L *L, int evenodd) {

@@ -4,6 +4,8 @@
 #pragma once
 #include "common.h"
 
+#include <functional>
+
 struct QPHIX_D3_ColorVector_struct : qphix::VectorT<double> {};
 struct QPHIX_F3_ColorVector_struct : qphix::VectorT<float> {};
 struct QPHIX_D3_FermionLinksAsqtad_struct : qphix::LinksT<double> {};
@@ -29,5 +31,11 @@ template <> struct Handles<float> {
 };
 
 template <typename T> void links_free(LinksT<T> *L);
+// A link handle built by the import pipeline of asqtad_create_L_from_raw from
+// raw MILC-order blocks that fill(cb, stage) writes on the device (fat, fat
+// back, long, long back at stage + k * 72 vh); null (error recorded) when
+// memory runs out or fill fails (fields.hip).
+template <typename T>
+typename Handles<T>::L *create_L_device(int evenodd, const std::function<bool(int, T *)> &fill);
 int long_recon_mode();
 } // namespace qphix

@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import lib as _L
-from .lib import EVEN, ODD, EVENODD, Info, InvertArg, ResidArg, GaugeCoeffs, ptr
+from .lib import EVEN, ODD, EVENODD, AsqtadCoeffs, Info, InvertArg, ResidArg, GaugeCoeffs, ptr
 
 _NP = {"D": np.float64, "F": np.float32}
 _CR = {"D": C.c_double, "F": C.c_float}
@@ -95,6 +95,24 @@ class Links:
             self.h = None
 
 
+class LinksFromGauge(Links):
+    """Fermion links built on the device from a Gauge field:
+    QPHIX_*3_asqtad_create_L_from_G (coeffs = (one_link, naik, three_staple,
+    five_staple, seven_staple, lepage)) or QPHIX_*3_hisq_create_L_from_G."""
+
+    def __init__(self, G, coeffs=None, hisq=False, naik_eps=0.0, periodic_t=False):
+        self.s, self.prec, self.parity = G.s, G.prec, EVENODD
+        self.info = Info()
+        flags = 1 if periodic_t else 0
+        if hisq:
+            self.h = G.s.fn(G.prec, "hisq_create_L_from_G")(C.byref(self.info), G.h, naik_eps, flags)
+        else:
+            c = AsqtadCoeffs(*coeffs)
+            self.h = G.s.fn(G.prec, "asqtad_create_L_from_G")(C.byref(self.info), C.byref(c), G.h, flags)
+        if not self.h:
+            raise RuntimeError("create_L_from_G failed: %s" % (G.s.lib.last_error(),))
+
+
 class Gauge:
     """QPHIX_*3_GaugeField from a raw 72-real/site array (all links)."""
 
@@ -161,6 +179,13 @@ def gauge_md(G, Fm, coeffs, eps, nsteps, scheme=0):
 
 def dslash(L, out, inp, parity):
     L.s.fn(L.prec, "asqtad_dslash")(L.h, out.h, inp.h, parity)
+
+
+def dslash_faces(L, out, inp, parity, cut_mask, reps=0):
+    """QPHIX_*3_asqtad_dslash_faces: the Dslash with the lattice cut along the
+    directions of cut_mask, faces exchanged by self-loopback (average ms when
+    reps > 0)."""
+    return L.s.fn(L.prec, "asqtad_dslash_faces")(L.h, out.h, inp.h, parity, cut_mask, reps)
 
 
 def dslash_multi(L, outs, inps, parity, reps=0):

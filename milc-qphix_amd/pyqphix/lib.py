@@ -35,6 +35,11 @@ class GaugeCoeffs(C.Structure):
                 ("parallelogram", C.c_double), ("adjoint_plaquette", C.c_double)]
 
 
+class AsqtadCoeffs(C.Structure):
+    _fields_ = [("one_link", C.c_double), ("naik", C.c_double), ("three_staple", C.c_double),
+                ("five_staple", C.c_double), ("seven_staple", C.c_double), ("lepage", C.c_double)]
+
+
 class InvertArg(C.Structure):
     _fields_ = [("parity", C.c_int), ("max", C.c_int), ("restart", C.c_int),
                 ("nrestart", C.c_int), ("evenodd", C.c_int)]
@@ -104,6 +109,10 @@ class Lib:
                  [C.POINTER(Info), _dp, _dp, C.POINTER(GaugeCoeffs), R])
             _sig(lib, f"QPHIX_{P}3_asqtad_dslash", None, [_dp, _dp, _dp, C.c_int])
             _sig(lib, f"QPHIX_{P}3_update_gauge", None, [C.POINTER(Info), _dp, _dp, R])
+            _sig(lib, f"QPHIX_{P}3_asqtad_dslash_faces", C.c_double, [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int])
+            _sig(lib, f"QPHIX_{P}3_asqtad_create_L_from_G", _dp,
+                 [C.POINTER(Info), C.POINTER(AsqtadCoeffs), _dp, C.c_int])
+            _sig(lib, f"QPHIX_{P}3_hisq_create_L_from_G", _dp, [C.POINTER(Info), _dp, C.c_double, C.c_int])
             _sig(lib, f"QPHIX_{P}3_gauge_md", None,
                  [C.POINTER(Info), _dp, _dp, C.POINTER(GaugeCoeffs), R, C.c_int, C.c_int])
             _sig(lib, f"QPHIX_{P}3_asqtad_dslash_time", C.c_double, [_dp, _dp, _dp, C.c_int, C.c_int])

@@ -62,6 +62,7 @@ def lib():
         L.oracle_gauge_update.argtypes = [vp, vp, vp, C.c_double]
         L.oracle_matrix_exp.argtypes = [vp, vp]
         L.oracle_gauge_md.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int]
+        L.oracle_links_from_gauge.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_int, vp, vp]
         L.oracle_fermion_oprod_d.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
@@ -115,6 +116,23 @@ def gauge_md(dims, U_raw, mom_raw, coef, eps, nsteps, scheme=0):
     c = np.ascontiguousarray(coef, dtype=np.float64)
     lib().oracle_gauge_md(_p(d), _p(U), _p(mom), _p(c), eps, nsteps, scheme)
     return U, mom
+
+
+ASQTAD = (5.0 / 8, -1.0 / 24, 1.0 / 16, 1.0 / 64, 1.0 / 384, -1.0 / 16)
+FAT7 = (1.0 / 8, 0.0, 1.0 / 16, 1.0 / 64, 1.0 / 384, 0.0)
+
+
+def links_from_gauge(dims, U_raw, coef=ASQTAD, hisq=False, naik_eps=0.0, apbc=True):
+    """Phased forward fat and Naik links (raw 72-real/site arrays) of the thin
+    field U: coef = (one_link, naik, three_staple, five_staple, seven_staple,
+    lepage), or HISQ."""
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    U = np.ascontiguousarray(U_raw, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    fat = np.zeros_like(U)
+    lng = np.zeros_like(U)
+    lib().oracle_links_from_gauge(_p(d), _p(U), _p(c), int(hisq), naik_eps, int(apbc), _p(fat), _p(lng))
+    return fat, lng
 
 
 def _p(a):

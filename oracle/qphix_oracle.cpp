@@ -34,6 +34,7 @@
  */
 #include <cmath>
 #include <complex>
+#include <initializer_list>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -798,6 +799,177 @@ void oracle_gauge_md(const int *latsize, double *U, double *mom, const double *c
     }
     if (nsteps > 0) force(lam * eps);
   }
+}
+
+/* Fermion links from the thin gauge field, path by path (the product is
+ * multiplied link by link along each path, no nesting).  Paths for link
+ * (x, mu), with signed directions d != +-mu: the link itself (c1); staples
+ * [d, mu, -d] (c3); ordered [d, e, mu, -e, -d], e != +-d (c5); ordered
+ * [d, e, f, mu, -f, -e, -d], all distinct directions (c7); [d, d, mu, -d, -d]
+ * (cL); Naik [mu, mu, mu] (cN).  Coefficients in the physical sign convention
+ * of Orginos-Toussaint-Sugar / Lepage (fat7: 1/8, 1/16, 1/64, 1/384).  HISQ
+ * (Follana et al. 2007): fat7, U(3) projection W = V (V^dag V)^(-1/2) (here
+ * from a Jacobi eigen-decomposition of V^dag V), then {1, 1/16, 1/64, 1/384,
+ * cL -1/8} and Naik -(1 + eps)/24 of W.  Phases: MILC's eta (C11) and -1 on
+ * t-links whose path crosses the last time slice when apbc. */
+static M3 path_product(const double *U, const GLat &g, const int *x, const int *steps, int n) {
+  int y[4] = {x[0], x[1], x[2], x[3]};
+  M3 P = meye();
+  for (int k = 0; k < n; ++k) {
+    const int s = steps[k], d = (s > 0 ? s : -s) - 1;
+    if (s > 0) {
+      P = mmul(P, get_link(U, g, y, d));
+      y[d] += 1;
+    } else {
+      y[d] -= 1;
+      P = mmul(P, madj(get_link(U, g, y, d)));
+    }
+  }
+  return P;
+}
+
+static void smear_paths(const double *U, const GLat &g, const double *c, double *fat, double *lng) {
+  const double c1 = c[0], cn = c[1], c3 = c[2], c5 = c[3], c7 = c[4], cl = c[5];
+  const long V = 2 * g.vh;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long s = 0; s < V; ++s) {
+    // coordinates of raw site s
+    const long p = s / g.vh, i = s % g.vh;
+    int x[4];
+    long r = i / (g.L[0] / 2);
+    const int xh = (int)(i % (g.L[0] / 2));
+    x[1] = (int)(r % g.L[1]);
+    r /= g.L[1];
+    x[2] = (int)(r % g.L[2]);
+    x[3] = (int)(r / g.L[2]);
+    x[0] = 2 * xh + (int)((x[1] + x[2] + x[3] + p) & 1);
+    for (int mu = 0; mu < 4; ++mu) {
+      const int m = mu + 1;
+      M3 acc{};
+      auto add = [&](double w, std::initializer_list<int> st) {
+        if (w == 0) return;
+        int steps[8], n = 0;
+        for (int v : st) steps[n++] = v;
+        const M3 P = path_product(U, g, x, steps, n);
+        for (int e = 0; e < 9; ++e) acc.a[e] += w * P.a[e];
+      };
+      add(c1, {m});
+      for (int d = -4; d <= 4; ++d) {
+        if (d == 0 || d == m || d == -m) continue;
+        add(c3, {d, m, -d});
+        add(cl, {d, d, m, -d, -d});
+        for (int e = -4; e <= 4; ++e) {
+          if (e == 0 || e == m || e == -m || e == d || e == -d) continue;
+          add(c5, {d, e, m, -e, -d});
+          for (int f = -4; f <= 4; ++f) {
+            if (f == 0 || f == m || f == -m || f == d || f == -d || f == e || f == -e) continue;
+            add(c7, {d, e, f, m, -f, -e, -d});
+          }
+        }
+      }
+      for (int e = 0; e < 9; ++e) {
+        fat[72 * s + 18 * mu + 2 * e] = acc.a[e].real();
+        fat[72 * s + 18 * mu + 2 * e + 1] = acc.a[e].imag();
+      }
+      if (lng) {
+        const int st[3] = {m, m, m};
+        const M3 N = path_product(U, g, x, st, 3);
+        for (int e = 0; e < 9; ++e) {
+          lng[72 * s + 18 * mu + 2 * e] = cn * N.a[e].real();
+          lng[72 * s + 18 * mu + 2 * e + 1] = cn * N.a[e].imag();
+        }
+      }
+    }
+  }
+}
+
+/* Eigen-decomposition of a Hermitian 3x3 matrix by cyclic complex Jacobi
+ * rotations: H = Q diag(w) Q^dag. */
+static void herm_eig(M3 H, double w[3], M3 &Q) {
+  Q = meye();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = i + 1; j < 3; ++j) off += std::norm(H.a[3 * i + j]);
+    if (off < 1e-34) break;
+    for (int p = 0; p < 3; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        const cd hpq = H.a[3 * p + q];
+        const double a = std::abs(hpq);
+        if (a < 1e-300) continue;
+        const cd ph = hpq / a;  // H_pq = a e^{i phi}
+        const double hpp = H.a[4 * p].real(), hqq = H.a[4 * q].real();
+        const double theta = 0.5 * std::atan2(2 * a, hpp - hqq);
+        const double cs = std::cos(theta), sn = std::sin(theta);
+        // rotation R with columns p, q: R_pp = c, R_qp = s conj(ph), R_pq = -s ph, R_qq = c
+        M3 R = meye();
+        R.a[4 * p] = cs;
+        R.a[4 * q] = cs;
+        R.a[3 * q + p] = sn * std::conj(ph);
+        R.a[3 * p + q] = -sn * ph;
+        H = mmul(madj(R), mmul(H, R));
+        Q = mmul(Q, R);
+      }
+  }
+  for (int k = 0; k < 3; ++k) w[k] = H.a[4 * k].real();
+}
+
+static void u3_project(const GLat &g, double *Vf) {
+#pragma omp parallel for schedule(static)
+  for (long l = 0; l < 8 * g.vh; ++l) {
+    M3 V;
+    for (int e = 0; e < 9; ++e) V.a[e] = cd(Vf[18 * l + 2 * e], Vf[18 * l + 2 * e + 1]);
+    double w[3];
+    M3 Q;
+    herm_eig(mmul(madj(V), V), w, Q);
+    M3 D{};
+    for (int k = 0; k < 3; ++k) D.a[4 * k] = 1.0 / std::sqrt(w[k]);
+    const M3 Wm = mmul(V, mmul(Q, mmul(D, madj(Q))));
+    for (int e = 0; e < 9; ++e) {
+      Vf[18 * l + 2 * e] = Wm.a[e].real();
+      Vf[18 * l + 2 * e + 1] = Wm.a[e].imag();
+    }
+  }
+}
+
+static void fold_phases(const GLat &g, double *f, int hop, int apbc) {
+  for (long s = 0; s < 2 * g.vh; ++s) {
+    const long p = s / g.vh, i = s % g.vh;
+    int x[4];
+    long r = i / (g.L[0] / 2);
+    const int xh = (int)(i % (g.L[0] / 2));
+    x[1] = (int)(r % g.L[1]);
+    r /= g.L[1];
+    x[2] = (int)(r % g.L[2]);
+    x[3] = (int)(r / g.L[2]);
+    x[0] = 2 * xh + (int)((x[1] + x[2] + x[3] + p) & 1);
+    const int eta[4] = {(x[3] & 1) ? -1 : 1, ((x[3] + x[0]) & 1) ? -1 : 1, ((x[3] + x[0] + x[1]) & 1) ? -1 : 1, 1};
+    for (int mu = 0; mu < 4; ++mu) {
+      double sg = eta[mu];
+      if (mu == 3 && apbc && x[3] + hop >= g.L[3]) sg = -sg;
+      for (int e = 0; e < 18; ++e) f[72 * s + 18 * mu + e] *= sg;
+    }
+  }
+}
+
+/* coef = {one_link, naik, three_staple, five_staple, seven_staple, lepage};
+ * hisq != 0 ignores coef and builds HISQ links with Naik factor 1 + naik_eps.
+ * fat, lng: raw forward links (72 reals/site, even sites first), phased. */
+void oracle_links_from_gauge(const int *latsize, const double *U, const double *coef, int hisq, double naik_eps,
+                             int apbc, double *fat, double *lng) {
+  const GLat g = glat(latsize);
+  if (!hisq) {
+    smear_paths(U, g, coef, fat, lng);
+  } else {
+    std::vector<double> W(72 * 2 * g.vh);
+    const double fat7[6] = {1.0 / 8, 0, 1.0 / 16, 1.0 / 64, 1.0 / 384, 0};
+    smear_paths(U, g, fat7, W.data(), nullptr);
+    u3_project(g, W.data());
+    const double lvl2[6] = {1.0, -(1.0 + naik_eps) / 24.0, 1.0 / 16, 1.0 / 64, 1.0 / 384, -1.0 / 8};
+    smear_paths(W.data(), g, lvl2, fat, lng);
+  }
+  fold_phases(g, fat, 1, apbc);
+  fold_phases(g, lng, 3, apbc);
 }
 
 /* Lattice set-up; the reference's minimum-size checks qphix_misc.cpp:494-497. */
