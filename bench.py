@@ -309,30 +309,6 @@ def force_24(A, F, dims=(24, 24, 24, 24)):
     return out
 
 
-def _weak_links_gpu(lat, F, seed, eps):
-    """Forward fat (1 + 0.3 Gaussian) and Naik (exp(i eps H)) links with phases,
-    for lattices too large for the host generator: the matrix exponentials run
-    with torch on the GPU in chunks.  Synthetic input only."""
-    import torch
-    rng = np.random.default_rng(seed)
-    n = lat.V * 4
-    fat = F.near_identity(rng, n, 0.3).reshape(lat.V, 4, 3, 3)
-    lng = np.empty((n, 3, 3), dtype=np.complex128)
-    chunk = 1 << 22
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    for a in range(0, n, chunk):
-        b = min(n, a + chunk)
-        z = torch.randn((b - a, 3, 3), dtype=torch.complex128, device="cuda", generator=g)
-        h = 0.5 * (z + z.conj().transpose(-1, -2))
-        tr = torch.diagonal(h, dim1=-2, dim2=-1).sum(-1).real / 3.0
-        h = h - tr[:, None, None] * torch.eye(3, dtype=torch.complex128, device="cuda")
-        lng[a:b] = torch.linalg.matrix_exp(1j * eps * h).cpu().numpy()
-        if (a // chunk) % 8 == 0:
-            log(f"[bench]   links {b}/{n}")
-    lng = lng.reshape(lat.V, 4, 3, 3)
-    return F.fold_phases(lat, fat, 1), F.fold_phases(lat, lng, 3)
-
-
 def combined_48(A, F, dims=(48, 48, 48, 96), nrhs=4, nshift=16, mass=0.0102):
     """One GPU's share of a production lattice: 48^3x96 weak-field links, 16-shift
     mixed multi-shift on 4 Gaussian sources (FP64 reliable updates, 1e-10 per
@@ -340,7 +316,7 @@ def combined_48(A, F, dims=(48, 48, 48, 96), nrhs=4, nshift=16, mass=0.0102):
     the FP64 path (FP64 multi-shift per source + the same force)."""
     lat = F.Lattice(dims)
     log(f"[bench] combined {dims}: generating links")
-    fat, lng = _weak_links_gpu(lat, F, 51, 0.3)
+    fat, lng = F.weak_links_device(lat, 51, 0.3, progress=lambda m: log(f"[bench]   {m}"))
     log("[bench]   raw link arrays")
     raw = F.raw_link_set(lat, fat, lng)
     del fat, lng
@@ -411,6 +387,8 @@ def main():
                     help="run only the 24^4 gauge- and fermion-force measurement (prints its JSON)")
     ap.add_argument("--combined", action="store_true",
                     help="run only the 48^3x96 multi-shift + fermion-force case (prints its JSON)")
+    ap.add_argument("--no-combined", action="store_true",
+                    help="leave out the 48^3x96 multi-shift + fermion-force case")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one lattice on one GPU; --gpus {args.gpus} is not "
@@ -532,6 +510,8 @@ def main():
         extra["multishift_batched"] = batch_multishift(A, F)
     if not args.no_force:
         extra["forces_24"] = force_24(A, F)
+    if not args.no_combined:
+        extra["combined_48x96"] = combined_48(A, F)
     print(json.dumps(out), flush=True)
     return 0
 

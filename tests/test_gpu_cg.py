@@ -127,3 +127,40 @@ def test_multishift_matches_oracle(prec):
     assert info.final_flop == pytest.approx((1205.0 + 15.0 * len(masses)) * ret * lat.V)
     for o in outs + [vb, L]:
         o.free()
+
+
+@pytest.mark.timeout(600)
+def test_cg_32x4_matches_oracle():
+    """The CG benchmark case at full size (32^4 weak-field links, 12-real Naik
+    storage, m = 0.02 in m^2 - D_eo D_oe, 1e-10 true relative residual): the
+    library's FP64 CG against the oracle's restatement of ks_d_congrad_fn.cpp
+    on the same inputs -- iteration counts within 5%, both below the target,
+    the same stopping residual to CG's rounding-order sensitivity, solutions
+    within 1e-8."""
+    dims = (32, 32, 32, 32)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    fat, lng = F.make_links(lat, "weak", 1002, eps=0.3)
+    raw = F.raw_link_set(lat, fat, lng)
+    del fat, lng
+    L = A.Links(s, *raw)
+    assert L.recon == 12
+    orc = O.Oracle(dims, *raw)
+    del raw
+    mass, resid = 0.01, 1e-20
+    b = F.to_raw(F.gaussian_source(lat, 2002, parity=0))
+    vb = A.Vector(s, b, A.EVEN)
+    vx = A.Vector(s, None, A.EVEN)
+    print("[case] cg 32^4 FP64 vs oracle", flush=True)
+    ret, info, res = A.invert(L, vx, vb, mass, A.EVEN, resid, niter=5000, nrestart=10)
+    xo, io = orc.invert(b, mass, O.EVEN, resid, niter=5000, nrestart=10)
+    print(f"[case] iterations {res.final_iter} (oracle {io['final_iter']}), rsq {res.final_rsq:.3e} "
+          f"(oracle {io['final_rsq']:.3e})", flush=True)
+    assert info.status == 0
+    assert abs(res.final_iter - io["final_iter"]) <= max(1, 0.05 * io["final_iter"])
+    assert res.final_rsq < resid and io["final_rsq"] < resid
+    assert abs(np.log10(res.final_rsq / io["final_rsq"])) < 0.3
+    assert rel_l2(half(vx.extract(), lat, 0), half(xo, lat, 0)) < 1e-8
+    orc.close()
+    for o in (vb, vx, L):
+        o.free()
