@@ -137,6 +137,23 @@ double QPHIX_D3_asqtad_dslash_faces(QPHIX_D3_FermionLinksAsqtad *asqtad, QPHIX_D
 double QPHIX_F3_asqtad_dslash_faces(QPHIX_F3_FermionLinksAsqtad *asqtad, QPHIX_F3_ColorVector *out,
                                     QPHIX_F3_ColorVector *in, QPHIX_evenodd_t parity, int cut_mask, int reps);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  asqtad_dslash_faces (reps = 0) with its face buffer in
+ * a fixed 64 MiB device array per GPU, taken at QPHIX_init and held until
+ * the process ends: no hipMalloc, hipFree or allocating library call.
+ * Faces needing more than 64 MiB (sizeof(Real) * 36 * sum over cut mu of
+ * the sites of one parity in a slice orthogonal to mu) are rejected with
+ * QPHIX_FAIL and a recorded error before anything is launched.  Returns
+ * QPHIX_SUCCESS or QPHIX_FAIL.  All calls on one GPU share the array: calls
+ * that overlap in time (other streams, concurrently replayed graphs) race.
+ * With QPHIX_MI355X_set_sync(0) the call returns once queued on the
+ * library's stream. */
+int QPHIX_D3_asqtad_dslash_faces_noalloc(QPHIX_D3_FermionLinksAsqtad *asqtad, QPHIX_D3_ColorVector *out,
+                                         QPHIX_D3_ColorVector *in, QPHIX_evenodd_t parity, int cut_mask);
+int QPHIX_F3_asqtad_dslash_faces_noalloc(QPHIX_F3_FermionLinksAsqtad *asqtad, QPHIX_F3_ColorVector *out,
+                                         QPHIX_F3_ColorVector *in, QPHIX_evenodd_t parity, int cut_mask);
+
 /* ---- fermion links from the gauge field ---------------------------------------
  * Asqtad coefficients per path, in the physical sign convention (links
  * without staggered phases): fat link = one_link U + three_staple (each of 6

@@ -93,6 +93,9 @@ void *dev_alloc(size_t bytes);
 void *dev_try_alloc(size_t bytes);
 int fewsums_mode();
 void set_fewsums_mode(int mode);
+// Takes the device address of the no-allocation face scratch for `device`
+// (which must be current); once per device, at init.  False: error recorded.
+bool faces_scratch_setup(int device);
 
 // ---------------------------------------------------------------------------
 // Device field storage.

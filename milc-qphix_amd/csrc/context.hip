@@ -153,6 +153,7 @@ static QPHIX_status_t init_common(QPHIX_layout_t *layout) {
     fail(__FILE__, __LINE__, "QPHIX_init: hipSetDevice failed");
     return QPHIX_FAIL;
   }
+  if (!faces_scratch_setup(c.device)) return QPHIX_FAIL;
   hipDeviceProp_t prop;
   HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
   c.n_cu = prop.multiProcessorCount;

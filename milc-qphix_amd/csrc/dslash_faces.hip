@@ -34,6 +34,35 @@
 
 namespace qphix {
 
+// Scratch of the no-allocation entry point (QPHIX_*3_asqtad_dslash_faces_noalloc):
+// its face buffer is this file-scope device array, one copy per GPU, held
+// until the process ends.  In host code `scratch` names a host-side stand-in,
+// not device memory: host code uses only the address hipGetSymbolAddress
+// returned for the current device (g_scratch_dev), taken once per device at
+// QPHIX_init (faces_scratch_setup).  Every call on a GPU shares its array and
+// finds the previous call's faces there; the pack kernels overwrite every
+// face site before the boundary kernels read one, so nothing is cleared.
+constexpr size_t FACE_SCRATCH_BYTES = size_t(64) << 20;
+constexpr int FACE_SCRATCH_MAX_DEV = 64;
+static __device__ __align__(256) unsigned char scratch[FACE_SCRATCH_BYTES];
+static void *g_scratch_dev[FACE_SCRATCH_MAX_DEV] = {};
+
+bool faces_scratch_setup(int device) {
+  QPHIX_REQUIRE(device >= 0 && device < FACE_SCRATCH_MAX_DEV, "face scratch: device id out of range", false);
+  if (g_scratch_dev[device]) return true;
+  int cur = -1;
+  QPHIX_REQUIRE(hipGetDevice(&cur) == hipSuccess && cur == device,
+                "face scratch: the device is not current", false);
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+    (void)hipGetLastError();
+    fail(__FILE__, __LINE__, "face scratch: hipGetSymbolAddress failed");
+    return false;
+  }
+  g_scratch_dev[device] = p;
+  return true;
+}
+
 struct FaceGeo {
   int L[4];
   int vh, cut;
@@ -313,32 +342,42 @@ static void launch_faces(const LinksT<T> &L, int cb, T *out, const T *in, T *fac
   HIP_CHECK(hipGetLastError());
 }
 
+// Argument checks shared by both entry points (errors recorded).
 template <typename T>
-static double dslash_faces(LinksT<T> *L, VectorT<T> *out, VectorT<T> *in, int parity, int cut, int reps) {
-  QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called: asqtad_dslash_faces", -1.0);
-  QPHIX_REQUIRE(L && out && in, "asqtad_dslash_faces: null argument", -1.0);
+static bool faces_args_ok(LinksT<T> *L, VectorT<T> *out, VectorT<T> *in, int parity, int cut) {
+  QPHIX_REQUIRE(L && out && in, "asqtad_dslash_faces: null argument", false);
   QPHIX_REQUIRE(parity == QPHIX_EVEN || parity == QPHIX_ODD, "asqtad_dslash_faces: parity must be EVEN or ODD",
-                -1.0);
+                false);
   const int cb = parity == QPHIX_EVEN ? 0 : 1;
   QPHIX_REQUIRE(out->half[cb] && in->half[1 - cb] && L->fat[cb] && L->lng[cb],
-                "asqtad_dslash_faces: vectors or links of the needed parity missing", -1.0);
-  QPHIX_REQUIRE(cut >= 0 && cut <= 15, "asqtad_dslash_faces: cut mask must be within 0..15", -1.0);
+                "asqtad_dslash_faces: vectors or links of the needed parity missing", false);
   const Geometry &gg = ctx().g;
   const int Ls[4] = {gg.nx, gg.ny, gg.nz, gg.nt};
   for (int mu = 0; mu < 4; ++mu)
     QPHIX_REQUIRE(!((cut >> mu) & 1) || Ls[mu] >= 8,
                   "asqtad_dslash_faces: a cut direction needs an extent of at least 8 (two disjoint 3-deep faces)",
-                  -1.0);
+                  false);
+  return true;
+}
+
+template <typename T> static void run_faces(LinksT<T> *L, VectorT<T> *out, VectorT<T> *in, int cb, T *faces, int cut) {
+  if (L->recon == 12) launch_faces<T, 12>(*L, cb, out->half[cb], in->half[1 - cb], faces, cut);
+  else launch_faces<T, 18>(*L, cb, out->half[cb], in->half[1 - cb], faces, cut);
+}
+
+template <typename T>
+static double dslash_faces(LinksT<T> *L, VectorT<T> *out, VectorT<T> *in, int parity, int cut, int reps) {
+  QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called: asqtad_dslash_faces", -1.0);
+  QPHIX_REQUIRE(cut >= 0 && cut <= 15, "asqtad_dslash_faces: cut mask must be within 0..15", -1.0);
+  if (!faces_args_ok(L, out, in, parity, cut)) return -1.0;
+  const int cb = parity == QPHIX_EVEN ? 0 : 1;
   const FaceGeo g = face_geo(cut);
   T *faces = nullptr;
   if (face_sites(g) > 0) {
     faces = (T *)ws2_get(sizeof(T) * 6 * (size_t)face_sites(g));
     if (!faces) return -1.0;  // error recorded
   }
-  auto run = [&]() {
-    if (L->recon == 12) launch_faces<T, 12>(*L, cb, out->half[cb], in->half[1 - cb], faces, cut);
-    else launch_faces<T, 18>(*L, cb, out->half[cb], in->half[1 - cb], faces, cut);
-  };
+  auto run = [&]() { run_faces<T>(L, out, in, cb, faces, cut); };
   if (reps <= 0) {
     run();
     sync_if_needed();
@@ -359,11 +398,50 @@ static double dslash_faces(LinksT<T> *L, VectorT<T> *out, VectorT<T> *in, int pa
   return (double)ms / reps;
 }
 
+// The same Dslash with the face buffer in the device array `scratch`: no
+// device memory is allocated or freed, no library routine that allocates is
+// called, and nothing is timed.  Faces beyond FACE_SCRATCH_BYTES are rejected
+// before anything is launched.
+template <typename T>
+static int dslash_faces_noalloc(LinksT<T> *L, VectorT<T> *out, VectorT<T> *in, int parity, int cut) {
+  QPHIX_REQUIRE(require_init(), "QPHIX_init has not been called: asqtad_dslash_faces_noalloc", (int)QPHIX_FAIL);
+  QPHIX_REQUIRE(cut >= 0 && cut <= 15, "asqtad_dslash_faces_noalloc: cut mask must be within 0..15",
+                (int)QPHIX_FAIL);
+  const FaceGeo g = face_geo(cut);
+  const size_t need = sizeof(T) * 6 * (size_t)face_sites(g);
+  if (need > FACE_SCRATCH_BYTES) {
+    char msg[200];
+    std::snprintf(msg, sizeof(msg),
+                  "asqtad_dslash_faces_noalloc: the faces of this lattice and cut mask need %zu bytes, more than "
+                  "the %zu-byte scratch (use asqtad_dslash_faces)",
+                  need, FACE_SCRATCH_BYTES);
+    fail(__FILE__, __LINE__, msg);
+    return QPHIX_FAIL;
+  }
+  if (!faces_args_ok(L, out, in, parity, cut)) return QPHIX_FAIL;
+  const int dev = ctx().device;
+  QPHIX_REQUIRE(dev >= 0 && dev < FACE_SCRATCH_MAX_DEV && g_scratch_dev[dev],
+                "asqtad_dslash_faces_noalloc: no scratch address for this device (QPHIX_init failed?)",
+                (int)QPHIX_FAIL);
+  run_faces<T>(L, out, in, parity == QPHIX_EVEN ? 0 : 1, static_cast<T *>(g_scratch_dev[dev]), cut);
+  sync_if_needed();
+  return QPHIX_SUCCESS;
+}
+
 } // namespace qphix
 
 using namespace qphix;
 
 extern "C" {
+
+int QPHIX_D3_asqtad_dslash_faces_noalloc(QPHIX_D3_FermionLinksAsqtad *L, QPHIX_D3_ColorVector *out,
+                                         QPHIX_D3_ColorVector *in, QPHIX_evenodd_t parity, int cut_mask) {
+  return dslash_faces_noalloc<double>(L, out, in, parity, cut_mask);
+}
+int QPHIX_F3_asqtad_dslash_faces_noalloc(QPHIX_F3_FermionLinksAsqtad *L, QPHIX_F3_ColorVector *out,
+                                         QPHIX_F3_ColorVector *in, QPHIX_evenodd_t parity, int cut_mask) {
+  return dslash_faces_noalloc<float>(L, out, in, parity, cut_mask);
+}
 
 double QPHIX_D3_asqtad_dslash_faces(QPHIX_D3_FermionLinksAsqtad *L, QPHIX_D3_ColorVector *out,
                                     QPHIX_D3_ColorVector *in, QPHIX_evenodd_t parity, int cut_mask, int reps) {

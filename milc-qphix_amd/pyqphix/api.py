@@ -188,6 +188,12 @@ def dslash_faces(L, out, inp, parity, cut_mask, reps=0):
     return L.s.fn(L.prec, "asqtad_dslash_faces")(L.h, out.h, inp.h, parity, cut_mask, reps)
 
 
+def dslash_faces_noalloc(L, out, inp, parity, cut_mask):
+    """QPHIX_*3_asqtad_dslash_faces_noalloc: dslash_faces with the face buffer
+    in the library's fixed device scratch (no allocation); returns the status."""
+    return L.s.fn(L.prec, "asqtad_dslash_faces_noalloc")(L.h, out.h, inp.h, parity, cut_mask)
+
+
 def dslash_multi(L, outs, inps, parity, reps=0):
     """QPHIX_F3_asqtad_dslash_multi (reps=0) or its timing variant (average ms)."""
     assert L.prec == "F" and len(outs) == len(inps)

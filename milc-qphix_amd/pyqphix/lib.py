@@ -110,6 +110,7 @@ class Lib:
             _sig(lib, f"QPHIX_{P}3_asqtad_dslash", None, [_dp, _dp, _dp, C.c_int])
             _sig(lib, f"QPHIX_{P}3_update_gauge", None, [C.POINTER(Info), _dp, _dp, R])
             _sig(lib, f"QPHIX_{P}3_asqtad_dslash_faces", C.c_double, [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int])
+            _sig(lib, f"QPHIX_{P}3_asqtad_dslash_faces_noalloc", C.c_int, [_dp, _dp, _dp, C.c_int, C.c_int])
             _sig(lib, f"QPHIX_{P}3_asqtad_create_L_from_G", _dp,
                  [C.POINTER(Info), C.POINTER(AsqtadCoeffs), _dp, C.c_int])
             _sig(lib, f"QPHIX_{P}3_hisq_create_L_from_G", _dp, [C.POINTER(Info), _dp, C.c_double, C.c_int])
