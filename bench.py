@@ -315,17 +315,18 @@ def combined_48(A, F, dims=(48, 48, 48, 96), nrhs=4, nshift=16, mass=0.0102):
     shift), then the fermion-force outer products of all 64 solutions; against
     the FP64 path (FP64 multi-shift per source + the same force)."""
     lat = F.Lattice(dims)
-    log(f"[bench] combined {dims}: generating links")
-    fat, lng = F.weak_links_device(lat, 51, 0.3, progress=lambda m: log(f"[bench]   {m}"))
-    log("[bench]   raw link arrays")
-    raw = F.raw_link_set(lat, fat, lng)
-    del fat, lng
+
     sig = np.logspace(np.log10(4 * mass * mass), 0.0, nshift)
     masses = list(np.sqrt(sig) / 2)
     coef = list(1.0 / (1.0 + np.arange(nshift)))
     out = {"lattice": list(dims), "nrhs": nrhs, "nshift": nshift, "mass": mass,
            "sigma": [float(x) for x in sig]}
     with A.Session(dims) as s:
+        log(f"[bench] combined {dims}: generating links")
+        fat, lng = A.weak_links_on_device(s, lat, 51, 0.3)
+        log("[bench]   raw link arrays")
+        raw = F.raw_link_set(lat, fat, lng)
+        del fat, lng
         log("[bench]   links to the device")
         L = A.Links(s, *raw)
         del raw
@@ -389,6 +390,8 @@ def main():
                     help="run only the 48^3x96 multi-shift + fermion-force case (prints its JSON)")
     ap.add_argument("--no-combined", action="store_true",
                     help="leave out the 48^3x96 multi-shift + fermion-force case")
+    ap.add_argument("--only-cg", action="store_true",
+                    help="run only the 32^4 FP64 / mixed / FP32 CG comparison (prints its JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one lattice on one GPU; --gpus {args.gpus} is not "
@@ -413,6 +416,13 @@ def main():
     t = time.perf_counter()
     raw, src = make_fields(lat, F)
     log(f"[bench] fields ready in {time.perf_counter() - t:.1f}s")
+    if args.only_cg:
+        with A.Session(DIMS) as s:
+            L64 = A.Links(s, *raw, prec="D")
+            res = cg_compare(A, s, L64, lat, src, raw)
+            L64.free()
+        print(json.dumps({"cg": res}), flush=True)
+        return 0
 
     s = A.Session(DIMS)
     lib = s.lib

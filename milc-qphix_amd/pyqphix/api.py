@@ -155,6 +155,40 @@ class Force:
             self.h = None
 
 
+def weak_links_on_device(s, lat, seed, eps=0.3, fat_sigma=0.3, chunk=1 << 20):
+    """fields.make_links(kind="weak") for lattices too large for the host
+    eigensolver: fat = 1 + fat_sigma Gaussian (host RNG); long = exp(eps A)
+    with A = (z - z^dag)/2 traceless, z complex Gaussian -- the distribution of
+    exp(i eps H), H Gaussian Hermitian traceless -- formed by the library's own
+    gauge update (QPHIX_D3_update_gauge on unit links).  Same distribution as
+    make_links, not the same numbers.  Synthetic input only.  Returns the
+    forward links (complex (V, 4, 3, 3)) with the staggered phases and the
+    antiperiodic time boundary folded in."""
+    from . import fields as F
+    rng = np.random.default_rng(seed)
+    fat = F.near_identity(rng, lat.V * 4, fat_sigma).reshape(lat.V, 4, 3, 3)
+    mom = np.empty((lat.V, 4, 3, 3), dtype=np.complex128)
+    for a in range(0, lat.V, chunk):
+        b = min(lat.V, a + chunk)
+        mom[a:b] = F.antihermitian_traceless(rng, (b - a, 4))
+    Fm = Force(s, mom.view(np.float64).ravel())
+    del mom
+    one = np.zeros((lat.V, 4, 3, 3), dtype=np.complex128)
+    for k in range(3):
+        one[..., k, k] = 1.0
+    G = Gauge(s, one.view(np.float64).ravel())
+    del one
+    info = update_gauge(G, Fm, eps)
+    if info.status != 0:
+        raise RuntimeError("update_gauge failed: %s" % (s.lib.last_error(),))
+    Fm.free()
+    del Fm
+    lng = G.extract().view(np.complex128).reshape(lat.V, 4, 3, 3)
+    G.free()
+    del G
+    return F.fold_phases(lat, fat, 1), F.fold_phases(lat, lng, 3)
+
+
 def gauge_force(G, Fm, coeffs, eps):
     info = Info()
     c = GaugeCoeffs(coeffs[0], coeffs[1], coeffs[2], coeffs[3] if len(coeffs) > 3 else 0.0)

@@ -163,33 +163,6 @@ def make_links(lat, kind, seed, eps=0.3, fat_sigma=None, antiperiodic=True):
     return fat, lng
 
 
-def weak_links_device(lat, seed, eps=0.3, fat_sigma=0.3, progress=None):
-    """make_links(kind="weak") for lattices too large for the host eigensolver:
-    fat = 1 + fat_sigma Gaussian (host RNG), long = exp(i eps H) with H
-    Gaussian Hermitian traceless, the matrix exponentials computed with torch
-    on the GPU in chunks of 4M links.  Same distribution as make_links, not
-    the same numbers.  Synthetic input only."""
-    import torch
-    rng = np.random.default_rng(seed)
-    n = lat.V * 4
-    fat = near_identity(rng, n, fat_sigma).reshape(lat.V, 4, 3, 3)
-    lng = np.empty((n, 3, 3), dtype=np.complex128)
-    chunk = 1 << 22
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    eye = torch.eye(3, dtype=torch.complex128, device="cuda")
-    for a in range(0, n, chunk):
-        b = min(n, a + chunk)
-        z = torch.randn((b - a, 3, 3), dtype=torch.complex128, device="cuda", generator=g)
-        h = 0.5 * (z + z.conj().transpose(-1, -2))
-        tr = torch.diagonal(h, dim1=-2, dim2=-1).sum(-1).real / 3.0
-        h = h - tr[:, None, None] * eye
-        lng[a:b] = torch.linalg.matrix_exp(1j * eps * h).cpu().numpy()
-        if progress is not None and (a // chunk) % 8 == 0:
-            progress(f"links {b}/{n}")
-    lng = lng.reshape(lat.V, 4, 3, 3)
-    return fold_phases(lat, fat, 1), fold_phases(lat, lng, 3)
-
-
 def free_links(lat, c1, c3, antiperiodic=True):
     """Free field: fat = c1 * eta * bc * 1, long = c3 * eta * bc * 1."""
     one = np.broadcast_to(np.eye(3, dtype=np.complex128), (lat.V, 4, 3, 3)).copy()

@@ -35,7 +35,7 @@ def test_combined_48x96_full_size():
     s = lattice_session(DIMS)
     lat = F.Lattice(DIMS)
     print("[case] 48^3x96: generating links", flush=True)
-    fat, lng = F.weak_links_device(lat, 51, 0.3)
+    fat, lng = A.weak_links_on_device(s, lat, 51, 0.3)
     raw = F.raw_link_set(lat, fat, lng)
     del fat, lng
     L = A.Links(s, *raw)
