@@ -266,23 +266,35 @@ def force_24(A, F, dims=(24, 24, 24, 24)):
         del U
         Fm = A.Force(s, np.zeros(72 * lat.V))
         coef = F.symanzik_coeffs(6.0, 0.86)
-        A.gauge_force(G, Fm, coef, 0.02)
-        reps = 20
-        t = time.perf_counter()
-        for _ in range(reps):
+
+        def timed():
             A.gauge_force(G, Fm, coef, 0.02)
-        sec = (time.perf_counter() - t) / reps
-        # executed: 140 SU(3) products (216 flops) per link -- the 48 loops with
-        # the walks that end on the same link sharing its product (139), and U * S;
-        # naive: 181 products, every loop on its own
-        flop = 140 * 216.0 * 4 * lat.V
+            reps = 20
+            t = time.perf_counter()
+            for _ in range(reps):
+                A.gauge_force(G, Fm, coef, 0.02)
+            return (time.perf_counter() - t) / reps
+
+        # staple-product kernel (the default for SU(3) links): 12 products per
+        # link to form the 24 plaquette staples of every site, 57 per link to
+        # combine them (plaquettes, 3 rectangle shapes, parallelograms) and W =
+        # U S; loop-walk kernel: 139 products per link (walks ending on the same
+        # link share it) and U S; naive: 181 products, every loop on its own
+        sec = timed()
+        s.lib.QPHIX_MI355X_set_gauge_force_staples(0)
+        sec_walk = timed()
+        s.lib.QPHIX_MI355X_set_gauge_force_staples(1)
+        flop = 69 * 216.0 * 4 * lat.V
         naive = 181 * 216.0 * 4 * lat.V
-        out["gauge_force"] = {"ms": round(sec * 1e3, 3), "GFLOPs": round(flop / sec / 1e9, 1),
+        out["gauge_force"] = {"ms": round(sec * 1e3, 3), "kernel": "staple products",
+                              "GFLOPs": round(flop / sec / 1e9, 1),
                               "frac_fp64_peak": round(flop / sec / 78.6e12, 4),
-                              "flop_per_link": 140 * 216, "naive_flop_per_link": 181 * 216,
-                              "naive_equivalent_GFLOPs": round(naive / sec / 1e9, 1)}
-        log(f"[bench] gauge force {sec * 1e3:.2f} ms, {flop / sec / 1e9:.0f} GFLOP/s executed "
-            f"({naive / sec / 1e9:.0f} naive-equivalent)")
+                              "flop_per_link": 69 * 216, "naive_flop_per_link": 181 * 216,
+                              "naive_equivalent_GFLOPs": round(naive / sec / 1e9, 1),
+                              "loop_walk_kernel_ms": round(sec_walk * 1e3, 3),
+                              "loop_walk_flop_per_link": 140 * 216}
+        log(f"[bench] gauge force {sec * 1e3:.2f} ms (loop-walk kernel {sec_walk * 1e3:.2f} ms), "
+            f"{flop / sec / 1e9:.0f} GFLOP/s executed ({naive / sec / 1e9:.0f} naive-equivalent)")
         G.free()
         Fm.free()
 

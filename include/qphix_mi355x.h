@@ -45,6 +45,11 @@ void QPHIX_MI355X_set_long_recon(int mode);
  * else all 3 rows), 18 = always 3 rows.  Applies to fields created afterwards;
  * extract_G always returns the links as given. */
 void QPHIX_MI355X_set_gauge_recon(int mode);
+/* Gauge force kernel: 1 (default) = built from stored plaquette staples
+ * (57 SU(3) products per link; used when G keeps its two-row copy, i.e. its
+ * links are SU(3) to rounding, and the staple buffer fits in device memory),
+ * 0 = every loop walked link by link (140 products per link; any field). */
+void QPHIX_MI355X_set_gauge_force_staples(int on);
 /* Rows the gauge force reads per link of G: 12 or 18 reals. */
 int QPHIX_D3_gauge_links_recon(QPHIX_D3_GaugeField *G);
 int QPHIX_F3_gauge_links_recon(QPHIX_F3_GaugeField *G);

@@ -5,6 +5,18 @@
 namespace qphix {
 
 constexpr int BLAS_BLOCK = 256;
+// Blocks of the vector kernels that sum the previous kernel's partials
+// themselves (consumer-side reductions).  Every block reads all of them (at
+// 32^4, 2048 Dslash partials x 4 sums: 64 KB per block), so larger blocks read
+// fewer bytes but leave fewer wavefronts resident.  cons_block() threads per
+// block: 512 by default (QPHIX_CONS_BLOCK = 256 or 1024 in the environment),
+// compiled for up to CONS_BLOCK_MAX.  Measured at 32^4 (mixed CG, FP32 /
+// FP16-stored vectors): 256 -> 0.190 / 0.180 s, 512 -> 0.187 / 0.179 s; 1024
+// made the single-reduction update 36.5 us against 21.5 us (half the
+// wavefronts resident).
+constexpr int CONS_BLOCK_MAX = 1024;
+int cons_block();
+inline int cons_nb(int vh) { return (vh + cons_block() - 1) / cons_block(); }
 
 ReduceCtl reduce_ctl(SolverState *st, int op);
 double *reduce_result_ptr();  // where reductions without a state leave their sums

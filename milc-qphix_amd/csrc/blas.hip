@@ -6,6 +6,8 @@
 // calc_rsq2 / update_dest / update_pm / copyKS (ks_multicg_offset.cpp:69-184).
 #include "blas.h"
 
+#include <cstdlib>
+
 namespace qphix {
 
 template <typename T>
@@ -76,10 +78,19 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
 // and the iteration (fields no other block of this kernel reads); |r|^2 and
 // the relative-residue sum go to out_partials.
 template <typename T>
-__global__ void __launch_bounds__(BLAS_BLOCK)
+__global__ void __launch_bounds__(CONS_BLOCK_MAX)
     k_cg_update_c(T *x, const T *p, T *r, const T *ttt, int vh, SolverState *st,
                   const double *pkp_partials, int nb_pkp, double *out_partials, const int *gate) {
   if (!gate_open(gate)) return;
+  using C = typename Spin<T>::C;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  cplx<C> xv[3], pv[3], rv[3], tv[3];
+  if (i < vh) {  // requested before the partials are summed
+    Spin<T>::load(x, i, xv);
+    Spin<T>::load(p, i, pv);
+    Spin<T>::load(r, i, rv);
+    Spin<T>::load(ttt, i, tv);
+  }
   double pk[1];
   sum_partials<1>(pkp_partials, nb_pkp, pk);
   const double rsq = st->rsq;
@@ -90,16 +101,9 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     st->iteration++;
     st->a = a;
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double s[2] = {0.0, 0.0};
   if (i < vh) {
-    using C = typename Spin<T>::C;
     const C ac = (C)a;
-    cplx<C> xv[3], pv[3], rv[3], tv[3];
-    Spin<T>::load(x, i, xv);
-    Spin<T>::load(p, i, pv);
-    Spin<T>::load(r, i, rv);
-    Spin<T>::load(ttt, i, tv);
     double num = 0;
     for (int c = 0; c < 3; ++c) {
       xv[c].re = xv[c].re + ac * pv[c].re;
@@ -125,9 +129,16 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
 // when the reference's inner loop would end (every block then skips the
 // update, whichever value of the gate it read).
 template <typename T>
-__global__ void __launch_bounds__(BLAS_BLOCK)
+__global__ void __launch_bounds__(CONS_BLOCK_MAX)
     k_cg_axpy_c(const T *r, T *p, int vh, SolverState *st, const double *partials, int nb, int *gate) {
   if (!gate_open(gate)) return;
+  using C = typename Spin<T>::C;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  cplx<C> rv[3], pv[3];
+  if (i < vh) {  // requested before the partials are summed
+    Spin<T>::load(r, i, rv);
+    Spin<T>::load(p, i, pv);
+  }
   double v[2];
   sum_partials<2>(partials, nb, v);
   const double rsq = v[0];
@@ -149,13 +160,8 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     if (stop) *gate = 0;
   }
   if (stop) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
-  using C = typename Spin<T>::C;
   const C bc = (C)b;
-  cplx<C> rv[3], pv[3];
-  Spin<T>::load(r, i, rv);
-  Spin<T>::load(p, i, pv);
   for (int c = 0; c < 3; ++c) {
     pv[c].re = rv[c].re + bc * pv[c].re;
     pv[c].im = rv[c].im + bc * pv[c].im;
@@ -177,10 +183,19 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
 // The relative residue needs a reduction of the updated vectors, so this step
 // is used only when relresid is off.
 template <typename T>
-__global__ void __launch_bounds__(BLAS_BLOCK)
+__global__ void __launch_bounds__(CONS_BLOCK_MAX)
     k_cg_step_fs(T *x, T *p, T *r, const T *ttt, int vh, SolverState *st, const double *partials, int nb,
                  int *gate) {
   if (st->fs_gate == 0) return;  // written by the preceding kernel: uniform
+  using C = typename Spin<T>::C;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  cplx<C> xv[3], pv[3], rv[3], tv[3];
+  if (i < vh) {  // requested before the partials are summed
+    Spin<T>::load(x, i, xv);
+    Spin<T>::load(p, i, pv);
+    Spin<T>::load(r, i, rv);
+    Spin<T>::load(ttt, i, tv);
+  }
   double v[4];
   sum_partials<4>(partials, nb, v);
   const double pkp = v[0], c_tr = v[1], c_tt = v[2], actual = v[3];
@@ -202,15 +217,8 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     st->b = b;
     if (stop) *gate = 0;
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
-  using C = typename Spin<T>::C;
   const C ac = (C)a, bc = (C)b;
-  cplx<C> xv[3], pv[3], rv[3], tv[3];
-  Spin<T>::load(x, i, xv);
-  Spin<T>::load(p, i, pv);
-  Spin<T>::load(r, i, rv);
-  Spin<T>::load(ttt, i, tv);
   for (int c = 0; c < 3; ++c) {
     xv[c].re = xv[c].re + ac * pv[c].re;
     xv[c].im = xv[c].im + ac * pv[c].im;
@@ -350,6 +358,15 @@ template <typename T> void blas_cg_axpy(const T *r, T *p, const SolverState *st,
   HIP_CHECK(hipGetLastError());
 }
 
+int cons_block() {
+  static const int b = [] {
+    const char *e = std::getenv("QPHIX_CONS_BLOCK");
+    const int v = e ? std::atoi(e) : 512;
+    return (v == 256 || v == 1024) ? v : 512;
+  }();
+  return b;
+}
+
 // Second partials region: consumer-side reductions read one region while
 // the consuming kernel writes its own partials into the other.
 double *partials_b() { return ctx().partials + 2 * (size_t)ctx().max_blocks; }
@@ -357,18 +374,18 @@ double *partials_b() { return ctx().partials + 2 * (size_t)ctx().max_blocks; }
 template <typename T>
 void blas_cg_step_c(T *x, T *p, T *r, const T *ttt, SolverState *st, int *gate, int nb_pkp) {
   const int vh = ctx().g.vh;
-  hipLaunchKernelGGL(k_cg_update_c<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, x, p, r, ttt, vh, st,
-                     ctx().partials, nb_pkp, partials_b(), gate);
-  hipLaunchKernelGGL(k_cg_axpy_c<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, r, p, vh, st,
-                     partials_b(), nb(vh), gate);
+  hipLaunchKernelGGL(k_cg_update_c<T>, dim3(cons_nb(vh)), dim3(cons_block()), 0, ctx().stream, x, p, r, ttt, vh,
+                     st, ctx().partials, nb_pkp, partials_b(), gate);
+  hipLaunchKernelGGL(k_cg_axpy_c<T>, dim3(cons_nb(vh)), dim3(cons_block()), 0, ctx().stream, r, p, vh, st,
+                     partials_b(), cons_nb(vh), gate);
   HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 void blas_cg_step_fs(T *x, T *p, T *r, const T *ttt, SolverState *st, int *gate, int nb_pkp) {
   const int vh = ctx().g.vh;
-  hipLaunchKernelGGL(k_cg_step_fs<T>, dim3(nb(vh)), dim3(BLAS_BLOCK), 0, ctx().stream, x, p, r, ttt, vh, st,
-                     ctx().partials, nb_pkp, gate);
+  hipLaunchKernelGGL(k_cg_step_fs<T>, dim3(cons_nb(vh)), dim3(cons_block()), 0, ctx().stream, x, p, r, ttt, vh,
+                     st, ctx().partials, nb_pkp, gate);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -304,6 +304,11 @@ struct BatchGraph {
 
 // Second workspace, for drivers that call other solvers while holding theirs.
 void *ws2_get(size_t bytes);
+// The same, null without a recorded error when the memory is not there.
+void *ws2_try_get(size_t bytes);
+int gauge_force_staples_mode();
+template <typename T>
+bool gauge_force_staples(GaugeT<T> *G, ForceT<T> *F, const QPHIX_gauge_coeffs_t *c, double eps);
 
 // Symanzik force step mom <- mom - eps TA(W) queued on the library stream
 // (gauge_force.hip); info->status = QPHIX_SUCCESS once queued.

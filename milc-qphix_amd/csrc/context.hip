@@ -107,6 +107,22 @@ void *ws2_get(size_t bytes) {
   return c.ws2;
 }
 
+void *ws2_try_get(size_t bytes) {
+  Context &c = g_ctx;
+  if (bytes > c.ws2_bytes) {
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    dev_free(c.ws2);
+    c.ws2 = dev_try_alloc(bytes);
+    c.ws2_bytes = c.ws2 ? bytes : 0;
+  }
+  return c.ws2;
+}
+
+// Gauge force kernel: 1 (default) the staple-product kernel where the links
+// are SU(3) to rounding, 0 always the loop-walk kernel.
+static int g_gf_staples = 1;
+int gauge_force_staples_mode() { return g_gf_staples; }
+
 static QPHIX_status_t init_common(QPHIX_layout_t *layout) {
   Context &c = g_ctx;
   if (c.initialised) return QPHIX_SUCCESS;  // F and D share one lattice (C14)
@@ -238,6 +254,7 @@ void QPHIX_MI355X_set_sync(int on) { g_ctx.sync_calls = on != 0; }
 void QPHIX_MI355X_debug_fail_alloc(int after) { g_fail_alloc_after = after; }
 void QPHIX_MI355X_set_fewsums(int mode) { qphix::set_fewsums_mode(mode); }
 void QPHIX_MI355X_set_graphs(int on) { g_ctx.graphs = on != 0; }
+void QPHIX_MI355X_set_gauge_force_staples(int on) { g_gf_staples = on != 0; }
 
 void QPHIX_MI355X_sync(void) {
   if (g_ctx.initialised) HIP_CHECK(hipStreamSynchronize(g_ctx.stream));

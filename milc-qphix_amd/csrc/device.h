@@ -67,6 +67,9 @@ __device__ __forceinline__ void recon_row2(cplx<C> *U, C s) {
 //
 // double and float vectors are stored tile-major over 64 sites: colour c of
 // site i is complex element spin_cidx(i, c) = ((i / 64) * 3 + c) * 64 + i % 64.
+// (Measured and dropped: FP32 tiles with colours 0-1 packed into one 16-byte
+// word per site, 2 loads per gather instead of 3: the FP32-link Dslash at 32^4
+// 89 -> 93 us, the 16-bit-link one 62 -> 57 us.)
 // A wavefront's 64 consecutive sites then read or write one contiguous block
 // per colour (1 KB FP64, 512 B FP32), and a neighbour gather touches a third of
 // the cache lines the site-major [site][colour] order needs (FP64 Dslash at

@@ -25,10 +25,20 @@ namespace qphix {
 // preceding Dslash's partials; block 0 records pkp, alpha, the iteration and
 // the old |r|^2 (fields no other block of this kernel reads).
 template <typename P>
-__global__ void __launch_bounds__(BLAS_BLOCK)
+__global__ void __launch_bounds__(CONS_BLOCK_MAX)
     k_mx_step_c(float *xs, const P *p, float *rs, const float *Ap, int vh, SolverState *st,
                 const double *pkp_partials, int nb_pkp, double *out_partials, const int *gate) {
   if (!gate_open(gate)) return;
+  // the site's vectors are requested before the partials are summed, so both
+  // latencies overlap
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  cplx<float> xv[3], pv[3], rv[3], av[3];
+  if (i < vh) {
+    Spin<float>::load(xs, i, xv);
+    Spin<P>::load(p, i, pv);
+    Spin<float>::load(rs, i, rv);
+    Spin<float>::load(Ap, i, av);
+  }
   double pk[1];
   sum_partials<1>(pkp_partials, nb_pkp, pk);
   const double rsq = st->mx_rsq;
@@ -39,15 +49,9 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     st->mx_alpha = alpha;
     st->mx_rsq_old = rsq;
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double s[1] = {0.0};
   if (i < vh) {
     const float al = (float)alpha;
-    cplx<float> xv[3], pv[3], rv[3], av[3];
-    Spin<float>::load(xs, i, xv);
-    Spin<P>::load(p, i, pv);
-    Spin<float>::load(rs, i, rv);
-    Spin<float>::load(Ap, i, av);
     for (int c = 0; c < 3; ++c) {
       xv[c].re += al * pv[c].re;
       xv[c].im += al * pv[c].im;
@@ -67,9 +71,15 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
 // closes the gate when an update is due (every block then skips, whichever
 // value of the gate it read; the running maximum is updated idempotently).
 template <typename P>
-__global__ void __launch_bounds__(BLAS_BLOCK)
+__global__ void __launch_bounds__(CONS_BLOCK_MAX)
     k_mx_axpy_c(const float *rs, P *p, int vh, SolverState *st, const double *partials, int nb, int *gate) {
   if (!gate_open(gate)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  cplx<float> rv[3], pv[3];
+  if (i < vh) {  // requested before the partials are summed
+    Spin<float>::load(rs, i, rv);
+    Spin<P>::load(p, i, pv);
+  }
   double v[1];
   sum_partials<1>(partials, nb, v);
   const double rnew = v[0];
@@ -84,12 +94,8 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     if (update) *gate = 0;
   }
   if (update) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   const float be = (float)beta;
-  cplx<float> rv[3], pv[3];
-  Spin<float>::load(rs, i, rv);
-  Spin<P>::load(p, i, pv);
   for (int c = 0; c < 3; ++c) {
     pv[c].re = rv[c].re + be * pv[c].re;
     pv[c].im = rv[c].im + be * pv[c].im;
@@ -106,10 +112,18 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
 // rr; the reliable-update test of k_mx_axpy_c; p = r_s + beta p unless an
 // update is due.  One pass over the vectors and no second reduction per step.
 template <typename P>
-__global__ void __launch_bounds__(BLAS_BLOCK)
+__global__ void __launch_bounds__(CONS_BLOCK_MAX)
     k_mx_step_fs(float *xs, P *p, float *rs, const float *Ap, int vh, SolverState *st, const double *partials,
                  int nb, int *gate) {
   if (st->fs_gate == 0) return;  // written by the preceding kernel: uniform
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  cplx<float> xv[3], pv[3], rv[3], av[3];
+  if (i < vh) {  // requested before the partials are summed
+    Spin<float>::load(xs, i, xv);
+    Spin<P>::load(p, i, pv);
+    Spin<float>::load(rs, i, rv);
+    Spin<float>::load(Ap, i, av);
+  }
   double v[4];
   sum_partials<4>(partials, nb, v);
   const double pkp = v[0], c_tr = v[1], c_tt = v[2], rr = v[3];
@@ -130,14 +144,8 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     st->mx_maxr = maxr;
     if (update) *gate = 0;
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   const float al = (float)alpha, be = (float)beta;
-  cplx<float> xv[3], pv[3], rv[3], av[3];
-  Spin<float>::load(xs, i, xv);
-  Spin<P>::load(p, i, pv);
-  Spin<float>::load(rs, i, rv);
-  Spin<float>::load(Ap, i, av);
   for (int c = 0; c < 3; ++c) {
     xv[c].re += al * pv[c].re;
     xv[c].im += al * pv[c].im;
@@ -154,12 +162,22 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   Spin<P>::store(p, i, pv);
 }
 
-// x += x_s ; x_s = 0
-__global__ void __launch_bounds__(BLAS_BLOCK) k_mx_accum(double *x, float *xs, int n) {
+// x += x_s ; x_s = 0 (per site: the FP64 and FP32 tiles order their reals
+// differently)
+__global__ void __launch_bounds__(BLAS_BLOCK) k_mx_accum(double *x, float *xs, int vh) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  x[i] += (double)xs[i];
-  xs[i] = 0.f;
+  if (i >= vh) return;
+  cplx<double> xv[3];
+  cplx<float> sv[3];
+  Spin<double>::load(x, i, xv);
+  Spin<float>::load(xs, i, sv);
+  for (int c = 0; c < 3; ++c) {
+    xv[c].re += (double)sv[c].re;
+    xv[c].im += (double)sv[c].im;
+    sv[c] = cplx<float>{0.f, 0.f};
+  }
+  Spin<double>::store(x, i, xv);
+  Spin<float>::store(xs, i, sv);
 }
 
 // r_s = (float) r ; p = r_s ; x_s = 0
@@ -323,8 +341,8 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
   int *gate = &dst->active;
   const double msq_x4 = 4.0 * mass * mass;
   const int nb = (vh + BLAS_BLOCK - 1) / BLAS_BLOCK;
+  const int nbc = cons_nb(vh);
   const int nb_dslash = (vh + DSLASH_BLOCK - 1) / DSLASH_BLOCK;
-  const int nbn = (int)((nv + BLAS_BLOCK - 1) / BLAS_BLOCK);
   const char *de = std::getenv("QPHIX_MIXED_DELTA");
   const int batch = 16;
   const bool fs = (fewsums_mode() & 2) != 0;  // single-reduction sloppy steps
@@ -373,7 +391,7 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
     bool first = true;
     while (true) {
       // reliable update: x += x_s, r = b - M x in FP64
-      if (!first) hipLaunchKernelGGL(k_mx_accum, dim3(nbn), dim3(BLAS_BLOCK), 0, ctx().stream, x, xs, (int)nv);
+      if (!first) hipLaunchKernelGGL(k_mx_accum, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, x, xs, vh);
       DslashExtra e1;
       launch_dslash<double>(*L, other, tmp64, x, EPI_PLAIN, e1);
       DslashExtra e2;
@@ -407,14 +425,14 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
           for (int k = 0; k < batch; ++k) {
             if (fs) {
               sloppy_apply(Lf, parity, p, tmps, Ap, rs, -msq_x4, dst, gate, EPI_PKP_FS);
-              hipLaunchKernelGGL(k_mx_step_fs<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs,
+              hipLaunchKernelGGL(k_mx_step_fs<P>, dim3(nbc), dim3(cons_block()), 0, ctx().stream, xs, p, rs,
                                  Ap, vh, dst, ctx().partials, nb_dslash, gate);
             } else {
               sloppy_apply(Lf, parity, p, tmps, Ap, rs, -msq_x4, dst, gate, EPI_PKP);
-              hipLaunchKernelGGL(k_mx_step_c<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs,
+              hipLaunchKernelGGL(k_mx_step_c<P>, dim3(nbc), dim3(cons_block()), 0, ctx().stream, xs, p, rs,
                                  Ap, vh, dst, ctx().partials, nb_dslash, partials_b(), gate);
-              hipLaunchKernelGGL(k_mx_axpy_c<P>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, p, vh,
-                                 dst, partials_b(), nb, gate);
+              hipLaunchKernelGGL(k_mx_axpy_c<P>, dim3(nbc), dim3(cons_block()), 0, ctx().stream, rs, p, vh,
+                                 dst, partials_b(), nbc, gate);
             }
           }
         });

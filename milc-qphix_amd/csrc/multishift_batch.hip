@@ -122,47 +122,99 @@ __device__ __forceinline__ void msx_one(cplx<T> *xv, cplx<T> *pv, const cplx<T> 
 #ifndef MSX_GROUP
 #define MSX_GROUP 2
 #endif
-template <typename T, int NR>
+// Per-block copy of the coefficients the update reads, for every source r of
+// the chunk and shift j: (beta, alpha, zeta) of the pending steps 0 .. 2 and of
+// this step, and sdone[j] -- read from LDS instead of each lane loading its own
+// source's state for every shift.
+constexpr int MSX_NC = 3 * MSX_MAXDEFER + 1;
+constexpr int MSX_STAGE_BYTES = 48 * 1024;  // larger (NR x nshift): coefficients read from the states
+template <int NR> inline bool msx_staged(int nshift) {
+  return (size_t)NR * nshift * MSX_NC * sizeof(double) <= (size_t)MSX_STAGE_BYTES;
+}
+
+// Blocks of k_msx_update_pm: as many as are resident at once (occupancy API x
+// CUs), each stepping through the slots, so the coefficient copy is made once
+// per resident block.
+template <typename K> inline int msx_pm_grid(K kernel, int nslots, size_t lds) {
+  int bpc = 0;
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel, BLAS_BLOCK, lds));
+  if (bpc < 1) bpc = 1;
+  const int need = (nslots + BLAS_BLOCK - 1) / BLAS_BLOCK;
+  const int res = bpc * ctx().n_cu;
+  return need < res ? need : res;
+}
+
+template <typename T, int NR, bool STAGED>
 __global__ void __launch_bounds__(BLAS_BLOCK)
     k_msx_update_pm(T *xs, T *p, const T *rs, T *rprev, int vh, int nshift, const SolverState *st) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= vh * NR) return;
-  const int r = t % NR;
+  extern __shared__ double msx_cf[];  // STAGED: [NR][nshift][MSX_NC]
+  if constexpr (STAGED) {
+    const int n = NR * nshift * MSX_NC;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const int rr = k / (nshift * MSX_NC), rem = k - rr * nshift * MSX_NC;
+      const int j = rem / MSX_NC, c = rem - j * MSX_NC;
+      const SolverState &sr = st[rr];
+      const int q = c / 3, w = c - 3 * q;
+      double v;
+      if (c == MSX_NC - 1) v = (double)sr.sdone[j];
+      else if (q < MSX_MAXDEFER - 1) v = w == 0 ? sr.pbeta[q][j] : (w == 1 ? sr.palpha[q][j] : sr.pzeta[q][j]);
+      else v = w == 0 ? sr.beta_i[j] : (w == 1 ? sr.alpha[j] : sr.zeta_ip1[j]);
+      msx_cf[k] = v;
+    }
+    __syncthreads();
+  }
+  // grid-stride over the (site, source) slots: the stride is a multiple of NR,
+  // so a work-item's source r is the same for all its slots
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = t0 % NR;
   const SolverState &s = st[r];
   if (!s.active) return;
+  // coefficient w (0 beta, 1 alpha, 2 zeta) of shift j for pending step q
+  // (q = MSX_MAXDEFER - 1: this step)
+  auto coef = [&](int q, int j, int w) -> T {
+    if constexpr (STAGED) return (T)msx_cf[((size_t)r * nshift + j) * MSX_NC + 3 * q + w];
+    else if (q < MSX_MAXDEFER - 1) return (T)(w == 0 ? s.pbeta[q][j] : (w == 1 ? s.palpha[q][j] : s.pzeta[q][j]));
+    else return (T)(w == 0 ? s.beta_i[j] : (w == 1 ? s.alpha[j] : s.zeta_ip1[j]));
+  };
+  auto retired = [&](int j) -> bool {
+    if constexpr (STAGED) return msx_cf[((size_t)r * nshift + j) * MSX_NC + MSX_NC - 1] == 1.0;
+    else return s.sdone[j] == 1;
+  };
+  constexpr int NOW = MSX_MAXDEFER - 1;
   const size_t stride = (size_t)vh * NR;
+  const int nn = s.num_now, jl = s.j_low;
+  const int np = s.pend;
+  // num_now only shrinks, so the first pending step covers every shift of the later ones
+  const int lim = min(np ? s.pend_nn[0] : nn, nshift);
+  for (int t = t0; t < vh * NR; t += gridDim.x * blockDim.x) {
   cplx<T> rv[3];
   Spin<T>::load(rs, t, rv);
-  const int nn = s.num_now, jl = s.j_low;
   auto step1 = [&](int j) {
     cplx<T> xv[3], pv[3];
     Spin<T>::load(xs + 6 * stride * j, t, xv);
     Spin<T>::load(p + 6 * stride * j, t, pv);
-    msx_one<T>(xv, pv, rv, (T)s.beta_i[j], (T)s.alpha[j], (T)s.zeta_ip1[j]);
+    msx_one<T>(xv, pv, rv, coef(NOW, j, 0), coef(NOW, j, 1), coef(NOW, j, 2));
     Spin<T>::store(xs + 6 * stride * j, t, xv);
     Spin<T>::store(p + 6 * stride * j, t, pv);
   };
   if (s.defer) {
     Spin<T>::store(rprev + 6 * stride * s.pend, t, rv);
     step1(jl);
-    return;
+    continue;
   }
   // Shifts MSX_GROUP at a time: the group's x_j, p_j loads are issued before
   // any store, so a work-item's shifts are not one dependent load-store chain.
-  const int np = s.pend;
   cplx<T> rq[MSX_MAXDEFER - 1][3];
 #pragma unroll
   for (int q = 0; q < MSX_MAXDEFER - 1; ++q)
     if (q < np) Spin<T>::load(rprev + 6 * stride * q, t, rq[q]);
-  // num_now only shrinks, so the first pending step covers every shift of the later ones
-  const int lim = min(np ? s.pend_nn[0] : nn, nshift);
   for (int j0 = 0; j0 < lim; j0 += MSX_GROUP) {
     bool on[MSX_GROUP];
     cplx<T> xv[MSX_GROUP][3], pv[MSX_GROUP][3];
 #pragma unroll
     for (int g = 0; g < MSX_GROUP; ++g) {
       const int j = j0 + g;
-      on[g] = j < lim && s.sdone[j] != 1;  // retired earlier: no update
+      on[g] = j < lim && !retired(j);  // retired earlier: no update
       if (on[g]) {
         Spin<T>::load(xs + 6 * stride * j, t, xv[g]);
         Spin<T>::load(p + 6 * stride * j, t, pv[g]);
@@ -175,14 +227,14 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
       if (j != jl) {  // the pending steps in order (the base system took them already)
 #pragma unroll
         for (int q = 0; q < MSX_MAXDEFER - 1; ++q)
-          if (q < np && j < s.pend_nn[q])
-            msx_one<T>(xv[g], pv[g], rq[q], (T)s.pbeta[q][j], (T)s.palpha[q][j], (T)s.pzeta[q][j]);
+          if (q < np && j < s.pend_nn[q]) msx_one<T>(xv[g], pv[g], rq[q], coef(q, j, 0), coef(q, j, 1), coef(q, j, 2));
       }
       if (j < nn || j == jl)  // then this one
-        msx_one<T>(xv[g], pv[g], rv, (T)s.beta_i[j], (T)s.alpha[j], (T)s.zeta_ip1[j]);
+        msx_one<T>(xv[g], pv[g], rv, coef(NOW, j, 0), coef(NOW, j, 1), coef(NOW, j, 2));
       Spin<T>::store(xs + 6 * stride * j, t, xv[g]);
       Spin<T>::store(p + 6 * stride * j, t, pv[g]);
     }
+  }
   }
 }
 
@@ -274,10 +326,17 @@ __global__ void k_msx_resume(SolverState *s, const SolverState *t, double delta)
   if (delta > 0) s->mx_delta = delta;
 }
 
-// FP64 -> FP32 copy (the direction vectors at the end of the FP64 phase).
+// FP64 -> FP32 copy of n consecutive spinor slots (the direction vectors at
+// the end of the FP64 phase), slot by slot: the FP64 and FP32 tiles order
+// their reals differently.
 __global__ void __launch_bounds__(BLAS_BLOCK) k_msx_demote(float *dst, const double *src, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (float)src[i];
+  if (i >= n) return;
+  cplx<double> d[3];
+  Spin<double>::load(src, (int)i, d);
+  cplx<float> f[3];
+  for (int c = 0; c < 3; ++c) f[c] = cplx<float>{(float)d[c].re, (float)d[c].im};
+  Spin<float>::store(dst, (int)i, f);
 }
 
 // Resident grid of a lane-mapped kernel: CUs x resident blocks per CU (from
@@ -392,19 +451,28 @@ static void mrhs_launch(const LinksT<T> &L, int epi, MrhsArgsT<T> a) {
   }
 }
 
-// Traversal order of the multi-RHS kernels.  Default: site order.
-// QPHIX_SITE_ORDER="py,pz" walks (py x pz) patches through each XCD's t-slab
-// (ordered_site) when the lattice divides into them.  Measured at 32^3x64,
-// 8 sources: 16% fewer bytes fetched (2.47 -> 2.07 GB per launch) but the same
-// time (69 us per source), so it is off by default; in the single-source
-// Dslash the same order breaks the sequential link stream and is 4% slower.
+// Traversal order of the multi-RHS kernels.  Default (nz a multiple of 8):
+// XCD x takes the z-planes [x nz / 8, (x + 1) nz / 8) and walks them t-slice
+// by t-slice (SiteOrder on = 2), so the 8 XCDs advance through t together and
+// the +-3 t-slice gather windows stay small enough for the Infinity Cache.
+// Measured at 32^3x64, 8 sources x 12 shifts (batched solve 3.13 -> 3.00 s):
+// FP64 plain / p.Ap Dslash 898 / 943 -> 797 / 810 us, FP32 350 -> 346 us.
+// QPHIX_SITE_ORDER=t: each XCD sweeps a t-slab in site order (the single-
+// source kernels' order); QPHIX_SITE_ORDER="py,pz": (py x pz) patches through
+// each XCD's t-slab (fewer bytes fetched, no faster: it breaks the sequential
+// link stream).  Every order is a permutation of which lane computes which
+// site; per-block partial sums (and so rounding) follow the order.
 static SiteOrder site_order(int NR) {
   const Geometry &g = ctx().g;
   SiteOrder o{0, 4, 4, 0};
-  const char *e = std::getenv("QPHIX_SITE_ORDER");
-  if (!e || std::strchr(e, ',') == nullptr) return o;
-  std::sscanf(e, "%d,%d", &o.py, &o.pz);
   (void)NR;
+  const char *e = std::getenv("QPHIX_SITE_ORDER");
+  if (!e || std::strcmp(e, "z") == 0) {
+    if (g.nz % N_XCD == 0) o.on = 2;
+    return o;
+  }
+  if (std::strchr(e, ',') == nullptr) return o;
+  std::sscanf(e, "%d,%d", &o.py, &o.pz);
   if (o.py < 1 || o.pz < 1 || g.ny % o.py || g.nz % o.pz || g.nt % N_XCD) return o;
   o.tslab = g.nt / N_XCD;
   o.on = 1;

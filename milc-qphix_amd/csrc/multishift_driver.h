@@ -30,8 +30,16 @@ static void msx_batch(const LinksT<T> &Lk, int parity, int batch, T *p, T *xs, T
                        ctx().partials, nlog);
     hipLaunchKernelGGL(k_finalize_multi<NR>, dim3(1), dim3(FINALIZE_BLOCK), 0, ctx().stream,
                        ctx().partials, nbr, st, (int)SOP_MSX_RSQ2, k);
-    hipLaunchKernelGGL((k_msx_update_pm<T, NR>), dim3((vh * NR + BLAS_BLOCK - 1) / BLAS_BLOCK), dim3(BLAS_BLOCK),
-                       0, ctx().stream, xs, p, rs, rprev, vh, nshift, st);
+    if (msx_staged<NR>(nshift)) {
+      const size_t lds = (size_t)NR * nshift * MSX_NC * sizeof(double);
+      const dim3 g(msx_pm_grid(k_msx_update_pm<T, NR, true>, vh * NR, lds));
+      hipLaunchKernelGGL((k_msx_update_pm<T, NR, true>), g, dim3(BLAS_BLOCK), (unsigned)lds, ctx().stream, xs, p, rs,
+                         rprev, vh, nshift, st);
+    } else {
+      const dim3 g(msx_pm_grid(k_msx_update_pm<T, NR, false>, vh * NR, 0));
+      hipLaunchKernelGGL((k_msx_update_pm<T, NR, false>), g, dim3(BLAS_BLOCK), 0, ctx().stream, xs, p, rs, rprev,
+                         vh, nshift, st);
+    }
     hipLaunchKernelGGL(k_msx_commit, dim3(1), dim3(NR), 0, ctx().stream, st, NR);
   }
 }
@@ -240,9 +248,10 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
                            xs_d, vh, NR, r, nshift);
         settle(r, true_residual(r), delta);
       }
-      const size_t np = (size_t)nshift * nvr;
-      hipLaunchKernelGGL(k_msx_demote, dim3((unsigned)((np + BLAS_BLOCK - 1) / BLAS_BLOCK)), dim3(BLAS_BLOCK), 0,
-                         ctx().stream, p, p_d, np);
+      const size_t np = (size_t)nshift * nvr;  // reals
+      const size_t nslot = np / 6;             // spinor slots (site x source, every shift)
+      hipLaunchKernelGGL(k_msx_demote, dim3((unsigned)((nslot + BLAS_BLOCK - 1) / BLAS_BLOCK)), dim3(BLAS_BLOCK), 0,
+                         ctx().stream, p, p_d, nslot);
       HIP_CHECK(hipMemsetAsync(xs, 0, np * sizeof(float), ctx().stream));
       HIP_CHECK(hipStreamSynchronize(ctx().stream));
       dev_free(w64);

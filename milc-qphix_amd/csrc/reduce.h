@@ -290,11 +290,29 @@ constexpr int FINALIZE_BLOCK = 1024;
 // reduction sums its per-block partials ([NV][nb]) itself, in one fixed order,
 // so every block obtains the same value bit for bit and no separate finalize
 // launch (and its dependency gap) is needed.  Result valid in every thread.
+// The loads of ROWS rows (one row = one partial per thread and sum) are all
+// issued before the first is added, so a block pays the L2 latency once per
+// ROWS rows; each thread still adds its partials in index order.  8 loads in
+// flight per thread whatever NV (more spilled in the 4-sum update kernels).
 template <int NV> __device__ __forceinline__ void sum_partials(const double *partials, int nb, double *v) {
+  constexpr int SUM_ROWS = NV >= 8 ? 1 : 8 / NV;
   double s[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) s[k] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+  const int bd = blockDim.x;
+  int b = threadIdx.x;
+  for (; b + (SUM_ROWS - 1) * bd < nb; b += SUM_ROWS * bd) {
+    double t[NV][SUM_ROWS];
+#pragma unroll
+    for (int u = 0; u < SUM_ROWS; ++u)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) t[k][u] = partials[(size_t)k * nb + b + u * bd];
+#pragma unroll
+    for (int u = 0; u < SUM_ROWS; ++u)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) s[k] += t[k][u];
+  }
+  for (; b < nb; b += bd) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) s[k] += partials[(size_t)k * nb + b];
   }

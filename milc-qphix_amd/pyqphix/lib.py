@@ -140,6 +140,7 @@ class Lib:
         _sig(lib, "QPHIX_MI355X_device_info", C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_double)])
         _sig(lib, "QPHIX_MI355X_set_long_recon", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_set_gauge_recon", None, [C.c_int])
+        _sig(lib, "QPHIX_MI355X_set_gauge_force_staples", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_set_mrhs_lds", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_set_msx_fuse", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_set_sloppy_link_bits", None, [C.c_int])

@@ -80,6 +80,33 @@ def test_gauge_force_two_row_links():
     Fm.free()
 
 
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (8, 12, 4, 16)])
+@pytest.mark.parametrize("coef", [MILC_B6, (0.3, -0.2, 0.1), (0.0, 0.0, 0.1), (0.0, -0.2, 0.0)])
+def test_gauge_force_staples_match_walks(dims, coef):
+    """SU(3) links: the staple-product kernel (rectangles and parallelograms
+    formed from stored plaquette staples) against the loop-walk kernel on the
+    same two-row field -- every link of both parities, to rounding."""
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    raw_u = F.to_raw(F.gauge_field(lat, "weak", 26, eps=0.4))
+    m0 = F.to_raw(F.antihermitian_traceless(np.random.default_rng(27), (lat.V, 4)))
+    G = A.Gauge(s, raw_u)
+    assert s.fn("D", "gauge_links_recon")(G.h) == 12
+    got = {}
+    try:
+        for on in (0, 1):
+            s.lib.QPHIX_MI355X_set_gauge_force_staples(on)
+            Fm = A.Force(s, m0)
+            print(f"[case] gauge force staples={on} dims={dims} coef={coef}", flush=True)
+            A.gauge_force(G, Fm, coef, 0.05)
+            got[on] = Fm.extract() - m0
+            Fm.free()
+    finally:
+        s.lib.QPHIX_MI355X_set_gauge_force_staples(1)
+    assert np.linalg.norm(got[1] - got[0]) < 1e-13 * np.linalg.norm(got[0])
+    G.free()
+
+
 def test_gauge_force_single_precision():
     dims = (8, 8, 8, 8)
     s = lattice_session(dims)
