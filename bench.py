@@ -35,7 +35,7 @@ DIMS = (32, 32, 32, 32)
 
 def site_bytes(real_bytes, n_long, spinor_bytes):
     """Algorithmic bytes per output site of one single-parity Dslash."""
-    sign = 1 if n_long == 12 else 0
+    sign = 1 if n_long in (12, 8) else 0
     return real_bytes * (8 * 18 + 8 * n_long) + sign + 2 * spinor_bytes
 
 
@@ -127,7 +127,8 @@ def cpu_baseline(dims, raw, src, budget_s):
 
 def cg_compare(A, s, L64, lat, src, raw_links):
     """FP64 CG vs mixed-precision CG (FP32 and FP16-stored sloppy spinors, FP32
-    or 16-bit fixed-point sloppy links, FP64 reliable updates), each to a 1e-10
+    sloppy links with 8- or 12-real long links or 16-bit fixed-point sloppy
+    links, FP64 reliable updates), each to a 1e-10
     true relative residual (resid is the squared residual, 1e-20), m = 0.02 in
     M = m^2 - D_eo D_oe (API mass m/2)."""
     mass, resid = 0.01, 1e-20
@@ -137,11 +138,16 @@ def cg_compare(A, s, L64, lat, src, raw_links):
     out = {}
     bnorm2 = float(b[:6 * lat.vh] @ b[:6 * lat.vh])
     x64 = None
-    for name, bits, lbits in (("f64", None, None), ("mixed_f32", 32, 32), ("mixed_f16", 16, 32),
-                              ("mixed_f32_l16", 32, 16), ("mixed_f16_l16", 16, 16)):
+    # mixed_*: FP32 sloppy links with 8-real long links (the default); *_r12:
+    # 12-real long links; *_l16: 16-bit fixed-point links
+    for name, bits, lbits, recon in (("f64", None, None, 8), ("mixed_f32", 32, 32, 8),
+                                     ("mixed_f16", 16, 32, 8), ("mixed_f32_r12", 32, 32, 12),
+                                     ("mixed_f16_r12", 16, 32, 12), ("mixed_f32_l16", 32, 16, 12),
+                                     ("mixed_f16_l16", 16, 16, 12)):
         vx = A.Vector(s, None, A.EVEN, "D")
         if lbits is not None:
             s.lib.QPHIX_MI355X_set_sloppy_link_bits(lbits)
+        s.lib.QPHIX_MI355X_set_sloppy_long_recon(recon)
         t = time.perf_counter()
         if bits is None:
             ret, info, res = A.invert(L64, vx, vb, mass, A.EVEN, resid, niter=5000, nrestart=10)
@@ -150,6 +156,7 @@ def cg_compare(A, s, L64, lat, src, raw_links):
                                             niter=5000, nrestart=10)
         sec = time.perf_counter() - t
         s.lib.QPHIX_MI355X_set_sloppy_link_bits(32)
+        s.lib.QPHIX_MI355X_set_sloppy_long_recon(8)
         x = vx.extract()[:6 * lat.vh]
         rec = {"seconds": round(sec, 4), "iters": res.final_iter, "final_rsq": res.final_rsq,
                "status": info.status}
@@ -157,6 +164,8 @@ def cg_compare(A, s, L64, lat, src, raw_links):
             x64 = x
         else:
             rec["reliable_updates"] = res.final_restart
+            rec["sloppy_links"] = "FP32, long links as 8 reals" if (lbits == 32 and recon == 8) else \
+                ("FP32, long links as 12 reals" if lbits == 32 else "16-bit fixed point")
             rec["rel_err_vs_f64"] = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
             rec["speedup_vs_f64"] = round(out["f64"]["seconds"] / sec, 3)
         out[name] = rec
@@ -493,6 +502,16 @@ def main():
                            "GBs": round(rb16 * lat.vh / (ms16 * 1e-3) / 1e9, 1),
                            "bytes_per_site": rb16, "recon": L64.recon}
     log(f"[bench] 16-bit-link Dslash {ms16 * 1e3:.1f} us  {extra['dslash_l16']['GBs']} GB/s")
+
+    # The mixed solver's FP32 sloppy Dslash: FP32 fat links, long links in their
+    # 8-real form (recon8), FP32 vectors
+    A.dslash_sloppy(L64, vout32, vin32, A.EVEN, 32, args.warmup)
+    ms8 = A.dslash_sloppy(L64, vout32, vin32, A.EVEN, 32, args.steps)
+    rb8 = site_bytes(4, 8, 24)
+    extra["dslash_f32_long8"] = {"us": round(ms8 * 1e3, 2),
+                                 "GBs": round(rb8 * lat.vh / (ms8 * 1e-3) / 1e9, 1),
+                                 "bytes_per_site": rb8, "recon": 8}
+    log(f"[bench] FP32 Dslash, 8-real long links {ms8 * 1e3:.1f} us  {extra['dslash_f32_long8']['GBs']} GB/s")
 
     if not args.no_cg:
         extra["cg"] = cg_compare(A, s, L64, lat, src, raw)

@@ -106,7 +106,8 @@ bool faces_scratch_setup(int device);
 //
 // Asqtad links of output parity cb: for each direction mu the forward link
 // (slot 2mu+1) and the back link (slot 2mu) form a "pair" of NC complex numbers
-// (NC = 18, or 12 when only rows 0-1 of each long link are stored).  A pair is
+// (NC = 18, or 12 when only rows 0-1 of each long link are stored, or 8 for
+// the FP32 sloppy copy's long links in their 8-real form, recon8).  A pair is
 // cut into 16-byte chunks (1 complex double or 2 complex float).  LINK_TILE
 // consecutive sites form a tile (one wavefront's sites), and LINK_GROUP
 // consecutive tiles a group; inside a group chunk k of direction mu is stored
@@ -170,11 +171,13 @@ __host__ __device__ __forceinline__ size_t link_sites(int vh) {
 template <typename T> struct LinksT {
   T *fat[2] = {nullptr, nullptr};
   T *lng[2] = {nullptr, nullptr};
-  uint8_t *lsign[2] = {nullptr, nullptr};  // 12-recon row-2 sign, bit (2mu+fwd)
-  int recon = 18;                          // 18 or 12 (long links)
+  uint8_t *lsign[2] = {nullptr, nullptr};  // 12- and 8-real row-2 sign, bit (2mu+fwd)
+  int recon = 18;                          // 18 or 12 (long links); 8: FP32 sloppy copy only
   int parity = 0;                          // QPHIX_EVEN/ODD/EVENODD supplied
   LinksT<float> *sloppy = nullptr;         // lazily built FP32 copy (D3 only)
   LinksT<Q16> *sloppy16 = nullptr;         // lazily built 16-bit copy (D3 only)
+  LinksT<float> *sloppy8 = nullptr;        // lazily built FP32 copy, 8-real long links (D3 only)
+  bool no8 = false;                        // some long link has no 8-real form
 };
 
 template <typename T> struct VectorT {
@@ -260,6 +263,7 @@ bool links_to_q16(const LinksT<double> &Ld, LinksT<Q16> &Lq);
 // The lazily built sloppy copies of a D3 link set (fields.hip); null (error
 // recorded) when they cannot be allocated
 const LinksT<float> *sloppy_links(LinksT<double> *L);
+const LinksT<float> *sloppy_links_solver(LinksT<double> *L);
 const LinksT<Q16> *sloppy_links16(LinksT<double> *L);
 // Link bits of the mixed solvers' sloppy operator: 32 (default) or 16.
 int sloppy_link_bits();

@@ -61,6 +61,57 @@ __device__ __forceinline__ void recon_row2(cplx<C> *U, C s) {
   }
 }
 
+// A link U = s V (V in SU(3), s = +-1, so det U = s) from 8 reals:
+// q[0] = U01, q[1] = U02, q[2] = U10, q[3] = (tan(arg U00 / 2), tan(arg U20 / 2)).
+// Unitarity gives n = |U01|^2 + |U02|^2 = 1 - |U00|^2 = |U10|^2 + |U20|^2, so
+// |U00| = sqrt(1 - n) and |U20| = sqrt(n - |U10|^2); a half-angle tangent h
+// gives the unit phase ((1 - h^2), 2h) / (1 + h^2).  The 2x2 block solves the
+// orthogonality of column 0 to columns 1 and 2 together with the cofactor
+// identities of det U = s, conj(U01) = s (U12 U20 - U10 U22) and
+// conj(U02) = s (U10 U21 - U11 U20):
+//   U11 = -(conj(U00) U01 U10 + s conj(U20) conj(U02)) / n
+//   U21 =  (s conj(U10) conj(U02) - conj(U00) U01 U20) / n
+//   U12 =  (s conj(U20) conj(U01) - conj(U00) U02 U10) / n
+//   U22 = -(s conj(U10) conj(U01) + conj(U00) U02 U20) / n
+// Every term of a numerator is a product of two of the small off-diagonal
+// elements, so the quotient keeps FP32 relative precision as n shrinks; n = 0
+// (U block-diagonal) has no 8-real form and is rejected when links are packed.
+// Hardware reciprocal and square root (1 ulp): this form feeds sloppy solver
+// operators only.
+__device__ __forceinline__ void recon8(const cplx<float> *q, float s, cplx<float> *U) {
+  const cplx<float> a1 = q[0], a2 = q[1], b0 = q[2];
+  const float n = fmaf(a1.re, a1.re, fmaf(a1.im, a1.im, fmaf(a2.re, a2.re, a2.im * a2.im)));
+  const float ma = __builtin_amdgcn_sqrtf(fmaxf(1.f - n, 0.f));
+  const float mc = __builtin_amdgcn_sqrtf(fmaxf(n - fmaf(b0.re, b0.re, b0.im * b0.im), 0.f));
+  const float ha = q[3].re, hc = q[3].im;
+  const float ra = ma * __builtin_amdgcn_rcpf(fmaf(ha, ha, 1.f));
+  const float rc = mc * __builtin_amdgcn_rcpf(fmaf(hc, hc, 1.f));
+  const cplx<float> a0{fmaf(-ha, ha, 1.f) * ra, 2.f * ha * ra};
+  const cplx<float> c0{fmaf(-hc, hc, 1.f) * rc, 2.f * hc * rc};
+  const float inv = __builtin_amdgcn_rcpf(n);
+  auto mul = [](cplx<float> x, cplx<float> y) {
+    return cplx<float>{fmaf(x.re, y.re, -x.im * y.im), fmaf(x.re, y.im, x.im * y.re)};
+  };
+  auto cjmul = [](cplx<float> x, cplx<float> y) {  // conj(x) y
+    return cplx<float>{fmaf(x.re, y.re, x.im * y.im), fmaf(x.re, y.im, -x.im * y.re)};
+  };
+  const cplx<float> t1 = cjmul(a0, a1), t2 = cjmul(a0, a2);
+  const cplx<float> sa1{s * a1.re, -s * a1.im}, sa2{s * a2.re, -s * a2.im};
+  const cplx<float> p11 = mul(t1, b0), q11 = cjmul(c0, sa2);
+  const cplx<float> p21 = cjmul(b0, sa2), q21 = mul(t1, c0);
+  const cplx<float> p12 = cjmul(c0, sa1), q12 = mul(t2, b0);
+  const cplx<float> p22 = cjmul(b0, sa1), q22 = mul(t2, c0);
+  U[0] = a0;
+  U[1] = a1;
+  U[2] = a2;
+  U[3] = b0;
+  U[4] = cplx<float>{-(p11.re + q11.re) * inv, -(p11.im + q11.im) * inv};
+  U[5] = cplx<float>{(p12.re - q12.re) * inv, (p12.im - q12.im) * inv};
+  U[6] = c0;
+  U[7] = cplx<float>{(p21.re - q21.re) * inv, (p21.im - q21.im) * inv};
+  U[8] = cplx<float>{-(p22.re + q22.re) * inv, -(p22.im + q22.im) * inv};
+}
+
 // ---------------------------------------------------------------------------
 // Spinor (colour vector) storage traits.  Compute type is double for double
 // storage and float otherwise.

@@ -44,7 +44,16 @@ static void launch_one(const DslashKArgs<TL, SI, SO> &a, int nblocks) {
 template <typename TL, typename SI, typename SO>
 static void dispatch(const DslashKArgs<TL, SI, SO> &a, int recon, int epi, bool consumer = false) {
   const int nb = (a.g.vh + DSLASH_BLOCK - 1) / DSLASH_BLOCK;
-  if (recon == 12) {
+  if (recon == 8) {  // built for the FP32 sloppy operator only
+    if constexpr (std::is_same<TL, float>::value) {
+      if (epi == EPI_PLAIN) launch_one<TL, 8, SI, SO, EPI_PLAIN>(a, nb);
+      else if (epi == EPI_PKP) launch_one<TL, 8, SI, SO, EPI_PKP>(a, nb);
+      else if (epi == EPI_PKP_FS) launch_one<TL, 8, SI, SO, EPI_PKP_FS>(a, nb);
+      else launch_one<TL, 8, SI, SO, EPI_RESTART>(a, nb);
+    } else {
+      QPHIX_REQUIRE(false, "Dslash: 8-real long links are FP32 only");
+    }
+  } else if (recon == 12) {
     if (epi == EPI_PLAIN) launch_one<TL, 12, SI, SO, EPI_PLAIN>(a, nb);
     else if (epi == EPI_PKP) launch_one<TL, 12, SI, SO, EPI_PKP>(a, nb);
     else if (epi == EPI_PKP_FS) launch_one<TL, 12, SI, SO, EPI_PKP_FS>(a, nb);

@@ -63,7 +63,7 @@ __device__ __forceinline__ int ordered_site(const Geo &g, const SiteOrder &o, in
 template <typename TL, typename SI, typename SO> struct DslashKArgs {
   const typename vec16<TL>::type *fat;  // output parity's links, tile-major
   const typename vec16<TL>::type *lng;
-  const uint8_t *lsign;                 // 12-recon signs (bit 2mu+1 fwd, 2mu back)
+  const uint8_t *lsign;                 // 12- and 8-real signs (bit 2mu+1 fwd, 2mu back)
   const SI *in;
   SO *out;
   Geo g;
@@ -92,7 +92,8 @@ __device__ __forceinline__ int block_slot(int opt) {
 
 // Applies one direction's link pair (forward link on in(fwd), back link on
 // in(bwd)).  NC = 18: both links complete; NC = 12: rows 0-1 of each, row 2
-// rebuilt as sign * conj(r0 x r1).
+// rebuilt as sign * conj(r0 x r1); NC = 8 (FP32 only): each link's 8-real form
+// (recon8).
 // The arithmetic of one pair: acc += U_f vf + U_b vb, u the pair as loaded.
 // Shared by every kernel that must give the same sums bit for bit.
 template <int NC, typename TC>
@@ -101,6 +102,13 @@ __device__ __forceinline__ void pair_compute(const cplx<TC> *u, const cplx<TC> *
   if constexpr (NC == 18) {
     matvec_acc(u, vf, acc);
     matvec_acc(u + 9, vb, acc);
+  } else if constexpr (NC == 8) {
+    static_assert(std::is_same<TC, float>::value, "8-real links are FP32 only");
+    cplx<TC> w[9];
+    recon8(u, sg_f ? -1.f : 1.f, w);
+    matvec_acc(w, vf, acc);
+    recon8(u + 4, sg_b ? -1.f : 1.f, w);
+    matvec_acc(w, vb, acc);
   } else {
     cplx<TC> w[9];
 #pragma unroll
@@ -165,7 +173,7 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) QPHIX_DSLASH_OCC k_dslash(const 
     SiteNbr n;
     site_neighbours(a.g, i, a.cb, n);
     unsigned sg = 0;
-    if constexpr (NL == 12) sg = a.lsign[i];
+    if constexpr (NL == 12 || NL == 8) sg = a.lsign[i];
     // grouped tile-major links (link_chunk): chunk k of direction mu at
     // link_base + (mu * NCH + k) * LINK_CSTRIDE + i % LINK_TILE
     const typename vec16<TL>::type *fb = a.fat + link_base(i, NCH_F);

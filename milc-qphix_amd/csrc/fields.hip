@@ -176,6 +176,50 @@ __global__ void k_links_to_float(float *dst, const double *src, int vh, int nc_p
     }
 }
 
+// tan(arg z / 2) for the 8-real form (recon8), from the half-angle identities
+// tan(t/2) = sin t / (1 + cos t) = (1 - cos t) / sin t, each used where it does
+// not cancel; arg z = pi maps to 1e18 (phase -1 to FP32 rounding), z = 0 to 0.
+__device__ double half_tan(double x, double y) {
+  const double r = hypot(x, y);
+  if (r == 0) return 0;
+  const double h = x >= 0 ? y / (r + x) : (y == 0 ? 1e18 : (r - x) / y);
+  return fmin(fmax(h, -1e18), 1e18);
+}
+
+// FP64 12-real long links -> FP32 8-real form: per link U01, U02, U10 and the
+// half-angle tangents of U00 and U20 = s conj(U01 U12 - U02 U11) (s the stored
+// row-2 sign).  Links with n = |U01|^2 + |U02|^2 < 1e-10 have no usable 8-real
+// form; they are counted in *bad and the caller keeps the 12-real form.
+__global__ void k_links_to_float8(float *dst, const double *src, const uint8_t *sign, int vh,
+                                  unsigned *bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  unsigned nbad = 0;
+  for (int mu = 0; mu < 4; ++mu)
+    for (int fb = 0; fb < 2; ++fb) {  // fb 0: forward link (chunks 0-5), 1: back link (6-11)
+      double u[12];
+      for (int k = 0; k < 6; ++k) {
+        const double *c = src + link_chunk(mu, 6 * fb + k, i, 12) * 2;
+        u[2 * k] = c[0];
+        u[2 * k + 1] = c[1];
+      }
+      const double s = ((sign[i] >> (2 * mu + (fb ? 0 : 1))) & 1u) ? -1.0 : 1.0;
+      // U01 U12 - U02 U11
+      const double xr = u[2] * u[10] - u[3] * u[11] - (u[4] * u[8] - u[5] * u[9]);
+      const double xi = u[2] * u[11] + u[3] * u[10] - (u[4] * u[9] + u[5] * u[8]);
+      const double n = u[2] * u[2] + u[3] * u[3] + u[4] * u[4] + u[5] * u[5];
+      if (!(n >= 1e-10)) ++nbad;
+      const float v[8] = {(float)u[2], (float)u[3], (float)u[4], (float)u[5], (float)u[6], (float)u[7],
+                          (float)half_tan(u[0], u[1]), (float)half_tan(s * xr, -s * xi)};
+      for (int k = 0; k < 4; ++k) {
+        float *d = dst + link_chunk(mu, (4 * fb + k) / 2, i, 4) * 4 + 2 * (k % 2);
+        d[0] = v[2 * k];
+        d[1] = v[2 * k + 1];
+      }
+    }
+  if (nbad) atomicAdd(bad, nbad);
+}
+
 // FP64 links (double2 chunks) -> 16-bit fixed point (Q16 chunks), per site
 // and direction one pair of nc_pair complex elements (forward link, then back
 // link), each link with its own scale.
@@ -278,6 +322,11 @@ template <typename T> void links_free(LinksT<T> *L) {
     delete L->sloppy16;
     L->sloppy16 = nullptr;
   }
+  if (L->sloppy8) {
+    links_free(L->sloppy8);
+    delete L->sloppy8;
+    L->sloppy8 = nullptr;
+  }
 }
 template void links_free<double>(LinksT<double> *);
 template void links_free<float>(LinksT<float> *);
@@ -307,6 +356,45 @@ bool links_to_float(const LinksT<double> &Ld, LinksT<float> &Lf) {
   }
   HIP_CHECK(hipGetLastError());
   return true;
+}
+
+// FP32 copy with 8-real long links (fat links as links_to_float).  Returns 0 on
+// success, 1 when some long link has no 8-real form (nothing kept, no error),
+// -1 when out of device memory (error recorded).
+static int links_to_float8(const LinksT<double> &Ld, LinksT<float> &Lf) {
+  const int vh = ctx().g.vh;
+  Lf.recon = 8;
+  Lf.parity = Ld.parity;
+  unsigned *bad = (unsigned *)dev_alloc(sizeof(unsigned));
+  if (!bad) return -1;
+  HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(unsigned), ctx().stream));
+  for (int cb = 0; cb < 2; ++cb) {
+    if (!Ld.fat[cb]) continue;
+    Lf.fat[cb] = (float *)dev_alloc(sizeof(float) * 4 * 36 * link_sites(vh));
+    Lf.lng[cb] = (float *)dev_alloc(sizeof(float) * 4 * 2 * 8 * link_sites(vh));
+    Lf.lsign[cb] = (uint8_t *)dev_alloc((size_t)vh);
+    if (!Lf.fat[cb] || !Lf.lng[cb] || !Lf.lsign[cb]) {  // error recorded
+      HIP_CHECK(hipStreamSynchronize(ctx().stream));
+      links_free(&Lf);
+      dev_free(bad);
+      return -1;
+    }
+    hipLaunchKernelGGL(k_links_to_float, dim3(nblk(vh)), dim3(256), 0, ctx().stream, Lf.fat[cb],
+                       Ld.fat[cb], vh, 18);
+    hipLaunchKernelGGL(k_links_to_float8, dim3(nblk(vh)), dim3(256), 0, ctx().stream, Lf.lng[cb],
+                       Ld.lng[cb], Ld.lsign[cb], vh, bad);
+    HIP_CHECK(hipMemcpyAsync(Lf.lsign[cb], Ld.lsign[cb], (size_t)vh, hipMemcpyDeviceToDevice, ctx().stream));
+  }
+  HIP_CHECK(hipGetLastError());
+  unsigned nbad = 0;
+  HIP_CHECK(hipMemcpyAsync(&nbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, ctx().stream));
+  HIP_CHECK(hipStreamSynchronize(ctx().stream));
+  dev_free(bad);
+  if (nbad) {
+    links_free(&Lf);
+    return 1;
+  }
+  return 0;
 }
 
 bool links_to_q16(const LinksT<double> &Ld, LinksT<Q16> &Lq) {
@@ -346,6 +434,31 @@ const LinksT<float> *sloppy_links(LinksT<double> *L) {
     L->sloppy = s;
   }
   return L->sloppy;
+}
+
+static int g_sloppy_long_recon = 8;
+
+// The FP32 links of the mixed solver's sloppy operator: long links in their
+// 8-real form when set (the default), the FP64 links keep 12 reals and every
+// long link has that form; otherwise the FP32 copy of sloppy_links.
+const LinksT<float> *sloppy_links_solver(LinksT<double> *L) {
+  if (g_sloppy_long_recon == 8 && L->recon == 12 && !L->no8) {
+    if (!L->sloppy8) {
+      auto *s = new LinksT<float>;
+      const int rc = links_to_float8(*L, *s);
+      if (rc < 0) {
+        delete s;
+        return nullptr;
+      }
+      if (rc == 0) L->sloppy8 = s;
+      else {
+        delete s;
+        L->no8 = true;
+      }
+    }
+    if (L->sloppy8) return L->sloppy8;
+  }
+  return sloppy_links(L);
 }
 
 const LinksT<Q16> *sloppy_links16(LinksT<double> *L) {
@@ -857,6 +970,7 @@ QPHIX_FIELD_API(F, float)
 void QPHIX_MI355X_set_long_recon(int mode) { g_recon_mode = mode == 18 ? 18 : 0; }
 void QPHIX_MI355X_set_gauge_recon(int mode) { g_gauge_recon_mode = mode == 18 ? 18 : 0; }
 void QPHIX_MI355X_set_sloppy_link_bits(int bits) { g_sloppy_link_bits = bits == 16 ? 16 : 32; }
+void QPHIX_MI355X_set_sloppy_long_recon(int recon) { g_sloppy_long_recon = recon == 12 ? 12 : 8; }
 
 void QPHIX_D3_asqtad_links_backcheck(QPHIX_D3_FermionLinksAsqtad *L, double out[4]) {
   backcheck<double>(L, out);

@@ -322,7 +322,7 @@ static int invert_mixed(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_
   const size_t nv = 6 * (size_t)vh;
   const LinksT<LK> *Lfp = [&]() -> const LinksT<LK> * {
     if constexpr (std::is_same<LK, Q16>::value) return sloppy_links16(L);
-    else return sloppy_links(L);
+    else return sloppy_links_solver(L);
   }();
   const size_t psz = sizeof(P) == sizeof(HalfSite) ? sizeof(HalfSite) * (size_t)vh : nv * sizeof(float);
   char *ws = Lfp ? (char *)ws_get(2 * nv * sizeof(double) + 2 * psz + 3 * nv * sizeof(float) +
@@ -510,7 +510,7 @@ double QPHIX_D3_asqtad_dslash_sloppy(QPHIX_D3_FermionLinksAsqtad *L, QPHIX_F3_Co
     HIP_CHECK(hipEventRecord(e0, ctx().stream));
     for (int r = 0; r < reps; ++r) launch_dslash_q16(Lq, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);
   } else {
-    const LinksT<float> *Lfp = sloppy_links(L);
+    const LinksT<float> *Lfp = sloppy_links_solver(L);
     if (!Lfp) return -1.0;  // error recorded
     const LinksT<float> &Lf = *Lfp;
     launch_dslash<float>(Lf, cb, out->half[cb], in->half[1 - cb], EPI_PLAIN, ex);

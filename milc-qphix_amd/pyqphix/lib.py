@@ -144,6 +144,7 @@ class Lib:
         _sig(lib, "QPHIX_MI355X_set_mrhs_lds", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_set_msx_fuse", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_set_sloppy_link_bits", None, [C.c_int])
+        _sig(lib, "QPHIX_MI355X_set_sloppy_long_recon", None, [C.c_int])
         _sig(lib, "QPHIX_D3_asqtad_dslash_sloppy", C.c_double, [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int])
         _sig(lib, "QPHIX_MI355X_set_graphs", None, [C.c_int])
         _sig(lib, "QPHIX_MI355X_debug_fail_alloc", None, [C.c_int])

@@ -20,9 +20,11 @@ def true_rsq(orc, lat, x, b, mass, cb):
     return float(r @ r / (bh @ bh))
 
 
+@pytest.mark.parametrize("recon", [8, 12])
 @pytest.mark.parametrize("bits", [32, 16])
 @pytest.mark.parametrize("kind,mass", [("haar", 0.025), ("weak", 0.01)])
-def test_mixed_reaches_fp64_residual(bits, kind, mass):
+def test_mixed_reaches_fp64_residual(bits, kind, mass, recon):
+    """recon: long links of the FP32 sloppy operator (8 reals, the default, or 12)."""
     dims = (8, 8, 8, 8)
     s = lattice_session(dims)
     lat = F.Lattice(dims)
@@ -33,10 +35,14 @@ def test_mixed_reaches_fp64_residual(bits, kind, mass):
     b = F.to_raw(F.gaussian_source(lat, 91, parity=0))
     vb = A.Vector(s, b, A.EVEN)
     vx = A.Vector(s, None, A.EVEN)
-    print(f"[case] mixed bits={bits} kind={kind}", flush=True)
+    print(f"[case] mixed bits={bits} kind={kind} long recon={recon}", flush=True)
     resid = 1e-20
-    ret, info, res = A.invert_mixed(L, vx, vb, mass, A.EVEN, resid, sloppy_bits=bits,
-                                    niter=2000, nrestart=10)
+    s.lib.QPHIX_MI355X_set_sloppy_long_recon(recon)
+    try:
+        ret, info, res = A.invert_mixed(L, vx, vb, mass, A.EVEN, resid, sloppy_bits=bits,
+                                        niter=2000, nrestart=10)
+    finally:
+        s.lib.QPHIX_MI355X_set_sloppy_long_recon(8)
     x = vx.extract()
     assert info.status == 0
     assert res.final_rsq < resid

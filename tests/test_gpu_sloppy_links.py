@@ -44,6 +44,73 @@ def test_sloppy_dslash_against_oracle(recon):
         o.free()
 
 
+@pytest.mark.parametrize("kind", ["weak", "haar"])
+def test_sloppy_dslash_8real_long_links(kind):
+    """The FP32 sloppy operator with 8-real long links (U01, U02, U10 and two
+    half-angle phase tangents; the rest from unitarity and det = +-1) against
+    the FP64 oracle, beside the 12-real form: both within the FP32 Dslash bar,
+    and the two outputs differ, so the 8-real kernels ran."""
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    raw = [F.to_raw(a) for a in random_links(lat, kind, 1017)]
+    orc = O.Oracle(dims, *raw)
+    L = A.Links(s, *raw)
+    assert s.fn("D", "asqtad_links_recon")(L.h) == 12
+    src = F.to_raw(F.gaussian_source(lat, 8), np.float32)
+    vin = A.Vector(s, src, A.EVENODD, "F")
+    vout = A.Vector(s, None, A.EVENODD, "F")
+    try:
+        for parity, cb in ((A.EVEN, 0), (A.ODD, 1)):
+            ref = orc.dslash(half(src.astype(np.float64), lat, 1 - cb), cb)
+            out = {}
+            for recon in (12, 8):
+                print(f"[case] sloppy dslash kind={kind} parity={parity} long recon={recon}", flush=True)
+                s.lib.QPHIX_MI355X_set_sloppy_long_recon(recon)
+                assert A.dslash_sloppy(L, vout, vin, parity, 32) > 0
+                out[recon] = half(vout.extract(), lat, cb)
+            e12, e8 = rel_l2(out[12], ref), rel_l2(out[8], ref)
+            print(f"[case] rel err 12-real {e12:.2e}, 8-real {e8:.2e}", flush=True)
+            assert e12 < 1e-6 and e8 < 1e-6
+            assert not np.array_equal(out[8], out[12])
+    finally:
+        s.lib.QPHIX_MI355X_set_sloppy_long_recon(8)
+    for o in (L, vin, vout):
+        o.free()
+
+
+def test_sloppy_8real_falls_back_without_the_form():
+    """Unit long links (times phases) store 12 reals but have |U01|^2 +
+    |U02|^2 = 0, which the 8-real form cannot represent: the sloppy operator
+    keeps the 12-real form, so both settings give the same output, exact
+    against the oracle to FP32 rounding."""
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    fat, lng = F.free_links(lat, 0.75, 1.0)
+    raw = list(F.raw_link_set(lat, fat, lng))
+    orc = O.Oracle(dims, *raw)
+    L = A.Links(s, *raw)
+    assert s.fn("D", "asqtad_links_recon")(L.h) == 12
+    src = F.to_raw(F.gaussian_source(lat, 9), np.float32)
+    vin = A.Vector(s, src, A.EVENODD, "F")
+    vout = A.Vector(s, None, A.EVENODD, "F")
+    ref = orc.dslash(half(src.astype(np.float64), lat, 1), 0)
+    out = {}
+    try:
+        for recon in (8, 12):
+            print(f"[case] sloppy dslash unit long links, long recon={recon}", flush=True)
+            s.lib.QPHIX_MI355X_set_sloppy_long_recon(recon)
+            assert A.dslash_sloppy(L, vout, vin, A.EVEN, 32) > 0
+            out[recon] = half(vout.extract(), lat, 0)
+    finally:
+        s.lib.QPHIX_MI355X_set_sloppy_long_recon(8)
+    assert np.array_equal(out[8], out[12])
+    assert rel_l2(out[8], ref) < 1e-6
+    for o in (L, vin, vout):
+        o.free()
+
+
 def true_rsq(orc, lat, x, b, mass, cb):
     xh = half(x, lat, cb)
     bh = half(b, lat, cb)
