@@ -18,8 +18,10 @@
 // + c_r [sum_n Q_{mu,n}(x) Q_{mu,n}(x-mu)] U_mu(x-mu)           rectangles n,-mu,-mu,-n,mu
 // + c_r sum_n L_n(x+mu) Q_{mu,n}(x+n) L_{-n}(x+n)               rectangles n,n,-mu,-n,-n
 // + c_g sum_A L_A(x+mu) sum_{d _|_ mu,A} Q_{mu,d}(x+A) Q_{A,d}(x)   parallelograms A,d,-mu,-A,-d
-// with Q_{-a,d}(x) = Q_{a,d}(x-a)^dag.  57 SU(3) products per link instead of
-// the walk kernel's 140, plus 12 per link to form the staples.
+// with Q_{-a,d}(x) = Q_{a,d}(x-a)^dag.  A rectangle n, n, -mu, -n, -n and the
+// parallelograms with A = n begin with the same link, so each such group is
+// one bracket times L_n(x+mu): 51 SU(3) products per link (with W = U S)
+// instead of the walk kernel's 140, plus 12 per link to form the staples.
 //
 // Kernel 1 (k_gq_staples): one work-item per (site, mu, nu), both signs of
 // nu; the staples are stored as their rows 0-1 (they are SU(3)), complex-
@@ -110,6 +112,15 @@ template <bool DA, bool DB> __device__ __forceinline__ void m_muladd(const M3 &a
       r.re[3 * i + j] = sr;
       r.im[3 * i + j] = si;
     }
+}
+
+// a *= c
+__device__ __forceinline__ void m_scale(M3 &a, double c) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    a.re[e] *= c;
+    a.im[e] *= c;
+  }
 }
 
 // r = op(a) op(b)
@@ -333,14 +344,17 @@ __global__ void __launch_bounds__(GQ_BLOCK) k_gq_force(const GqArgs a) {
   const int r_pmu = q_move(g, r, cmu, mu, 1), r_mmu = q_move(g, r, cmu, mu, -1);
   const int i_pmu = r_pmu >> 1, i_mmu = r_mmu >> 1;  // parity 1 - cb
   __shared__ double sacc[18 * GQ_BLOCK];
-  double *S = sacc + threadIdx.x;
+  __shared__ double sgb[18 * GQ_BLOCK];
+  double *S = sacc + threadIdx.x, *GB = sgb + threadIdx.x;
 #pragma unroll
-  for (int e = 0; e < 18; ++e) S[e * GQ_BLOCK] = 0.0;
+  for (int e = 0; e < 18; ++e) S[e * GQ_BLOCK] = GB[e * GQ_BLOCK] = 0.0;
   M3 G, A, B;
   // The loops stay rolled: one term's matrices in registers at a time (the
   // latency is hidden by the other wavefronts), and every index is computed
   // from uniform directions, so nothing is indexed per lane.
-  // plaquettes and the rectangles that continue along mu
+  // Plaquettes and the rectangles along mu in one pass over n, each Q_{mu,n}(x)
+  // loaded once: G = sum_n Q_{mu,n}(x+mu) Q_{mu,n}(x) in registers, the
+  // other rectangle sum sum_n Q_{mu,n}(x) Q_{mu,n}(x-mu) in LDS (GB).
   m_zero(G);
 #pragma unroll 2
   for (int n = 0; n < 6; ++n) {
@@ -349,66 +363,63 @@ __global__ void __launch_bounds__(GQ_BLOCK) k_gq_force(const GqArgs a) {
     s_axpy(S, a.cp, A);
     load2(Qo, qn, vh, i_pmu, B);      // Q_{mu,n}(x+mu)
     m_muladd<false, false>(B, A, G);
+    load2(Qo, qn, vh, i_mmu, B);      // Q_{mu,n}(x-mu)
+    s_muladd<false, false>(GB, 1.0, A, B);
   }
   load2(Uo, mu, vh, i_pmu, A);        // U_mu(x+mu)
   s_muladd<false, false>(S, a.cr, A, G);
-  m_zero(G);
-#pragma unroll 2
-  for (int n = 0; n < 6; ++n) {
-    const int qn = q_slot(mu, q_other(mu, n >> 1), n & 1);
-    load2(Qs, qn, vh, i, A);          // Q_{mu,n}(x)
-    load2(Qo, qn, vh, i_mmu, B);      // Q_{mu,n}(x-mu)
-    m_muladd<false, false>(A, B, G);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    G.re[e] = GB[(2 * e) * GQ_BLOCK];
+    G.im[e] = GB[(2 * e + 1) * GQ_BLOCK];
   }
   load2(Uo, mu, vh, i_mmu, A);        // U_mu(x-mu)
   s_muladd<false, false>(S, a.cr, G, A);
-  // rectangles n, n, -mu, -n, -n: L_n(x+mu) Q_{mu,n}(x+n) L_{-n}(x+n)
+  // Per direction nu _|_ mu and sign, the rectangle n, n, -mu, -n, -n and the
+  // parallelograms that start with the same link L_n(x+mu) share it:
+  //   +nu: U_nu(x+mu) [c_r Q_{mu,+nu}(x+nu) U_nu(x)^dag + c_g sum_d Q_{mu,d}(x+nu) Q_{nu,d}(x)]
+  //   -nu: U_nu(x+mu-nu)^dag [c_r Q_{mu,-nu}(x-nu) U_nu(x-nu) + c_g sum_d Q_{mu,d}(x-nu) Q_{nu,d}(x-nu)^dag]
+  // (d over the four signed directions orthogonal to mu and nu).
 #pragma unroll 1
   for (int k = 0; k < 3; ++k) {
     const int nu = q_other(mu, k), cn = q_coord(c, nu);
     const int i_p = q_move(g, r, cn, nu, 1) >> 1, i_m = q_move(g, r, cn, nu, -1) >> 1;  // parity 1 - cb
     const int i_mo = q_move(g, r_pmu, cn, nu, -1) >> 1;                                 // x+mu-nu, parity cb
-    // n = +nu: U_nu(x+mu) Q(x+nu) U_nu(x)^dag
-    load2(Uo, nu, vh, i_pmu, A);
-    load2(Qo, q_slot(mu, nu, false), vh, i_p, B);
-    m_mul<false, false>(A, B, G);
-    load2(Us, nu, vh, i, A);
-    s_muladd<false, true>(S, a.cr, G, A);
-    // n = -nu: U_nu(x+mu-nu)^dag Q(x-nu) U_nu(x-nu)
-    load2(Us, nu, vh, i_mo, A);
-    load2(Qo, q_slot(mu, nu, true), vh, i_m, B);
-    m_mul<true, false>(A, B, G);
-    load2(Uo, nu, vh, i_m, A);
-    s_muladd<false, false>(S, a.cr, G, A);
-  }
-  // parallelograms A, d, -mu, -A, -d: L_A(x+mu) sum_d Q_{mu,d}(x+A) Q_{A,d}(x)
-#pragma unroll 1
-  for (int ka = 0; ka < 3; ++ka) {
-    const int ad = q_other(mu, ka), ca = q_coord(c, ad);
-    const int i_pa = q_move(g, r, ca, ad, 1) >> 1, i_ma = q_move(g, r, ca, ad, -1) >> 1;  // parity 1 - cb
-    const int i_mo = q_move(g, r_pmu, ca, ad, -1) >> 1;                                   // x+mu-a, parity cb
-    // A = +a: U_a(x+mu) sum_d Q_{mu,d}(x+a) Q_{a,d}(x)
+    // +nu
     m_zero(G);
 #pragma unroll 2
     for (int j = 0; j < 4; ++j) {
-      const int dd = q_other(mu, (ka + 1 + (j >> 1)) % 3);
-      load2(Qo, q_slot(mu, dd, j & 1), vh, i_pa, A);
-      load2(Qs, q_slot(ad, dd, j & 1), vh, i, B);
+      const int dd = q_other(mu, (k + 1 + (j >> 1)) % 3);
+      load2(Qo, q_slot(mu, dd, j & 1), vh, i_p, A);
+      load2(Qs, q_slot(nu, dd, j & 1), vh, i, B);
       m_muladd<false, false>(A, B, G);
     }
-    load2(Uo, ad, vh, i_pmu, A);
-    s_muladd<false, false>(S, a.cg, A, G);
-    // A = -a: U_a(x+mu-a)^dag sum_d Q_{mu,d}(x-a) Q_{a,d}(x-a)^dag
+    m_scale(G, a.cg);
+    load2(Qo, q_slot(mu, nu, false), vh, i_p, A);
+    load2(Us, nu, vh, i, B);
+    m_scale(A, a.cr);
+    m_muladd<false, true>(A, B, G);
+    load2(Uo, nu, vh, i_pmu, A);
+    s_muladd<false, false>(S, 1.0, A, G);
+    // keeps the -nu loads from being hoisted into the +nu half (without it the
+    // kernel takes 256 VGPRs, one wavefront per SIMD; with it 210, two)
+    __builtin_amdgcn_sched_barrier(0);
+    // -nu
     m_zero(G);
 #pragma unroll 2
     for (int j = 0; j < 4; ++j) {
-      const int dd = q_other(mu, (ka + 1 + (j >> 1)) % 3);
-      load2(Qo, q_slot(mu, dd, j & 1), vh, i_ma, A);
-      load2(Qo, q_slot(ad, dd, j & 1), vh, i_ma, B);
+      const int dd = q_other(mu, (k + 1 + (j >> 1)) % 3);
+      load2(Qo, q_slot(mu, dd, j & 1), vh, i_m, A);
+      load2(Qo, q_slot(nu, dd, j & 1), vh, i_m, B);
       m_muladd<false, true>(A, B, G);
     }
-    load2(Us, ad, vh, i_mo, A);
-    s_muladd<true, false>(S, a.cg, A, G);
+    m_scale(G, a.cg);
+    load2(Qo, q_slot(mu, nu, true), vh, i_m, A);
+    load2(Uo, nu, vh, i_m, B);
+    m_scale(A, a.cr);
+    m_muladd<false, false>(A, B, G);
+    load2(Us, nu, vh, i_mo, A);
+    s_muladd<true, false>(S, 1.0, A, G);
   }
   // W = U_mu(x) S; H <- H + i eps TA(W) (momentum stored as H = -iA, 8 reals)
   {
