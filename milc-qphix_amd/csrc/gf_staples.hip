@@ -272,7 +272,6 @@ __global__ void __launch_bounds__(256) k_gq_staples(const T *u0, const T *u1, T 
 
 struct GqArgs {
   const void *u12[2];   // two-row links, parity 0 / 1
-  const void *u[2];     // all rows (the force's own link)
   const void *q[2];     // staples
   void *h[2];           // momentum
   GqGeo g;
@@ -422,16 +421,7 @@ __global__ void __launch_bounds__(GQ_BLOCK) k_gq_force(const GqArgs a) {
     s_muladd<true, false>(S, 1.0, A, G);
   }
   // W = U_mu(x) S; H <- H + i eps TA(W) (momentum stored as H = -iA, 8 reals)
-  {
-    using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
-    const T2 *u = reinterpret_cast<const T2 *>(a.u[cb]);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) {
-      const T2 v = u[gauge_elem(mu, e, i, vh)];
-      A.re[e] = (double)v.x;
-      A.im[e] = (double)v.y;
-    }
-  }
+  load2(Us, mu, vh, i, A);            // U_mu(x), row 2 rebuilt like every other link here
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
     B.re[e] = S[(2 * e) * GQ_BLOCK];
@@ -440,18 +430,30 @@ __global__ void __launch_bounds__(GQ_BLOCK) k_gq_force(const GqArgs a) {
   M3 &W = G;
   m_mul<false, false>(A, B, W);
   const double imtr = (W.im[0] + W.im[4] + W.im[8]) / 3.0;
+  // the link's 8 momentum reals (32 or 64 bytes, 16-byte aligned) as 16-byte
+  // loads and stores: lanes are 4 links apart, so each access touches one
+  // cache line per lane
   T *h = static_cast<T *>(a.h[cb]) + 32 * (size_t)i + 8 * mu;
+  using V16 = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+  constexpr int NV = 8 * sizeof(T) / 16;
+  V16 *hv = reinterpret_cast<V16 *>(h);
+  V16 hr[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) hr[k] = hv[k];
+  T *hh = reinterpret_cast<T *>(hr);
   const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
 #pragma unroll
   for (int qq = 0; qq < 3; ++qq) {
     const int rr = pr[qq][0], cc = pr[qq][1];
     const double tre = 0.5 * (W.re[3 * rr + cc] - W.re[3 * cc + rr]);
     const double tim = 0.5 * (W.im[3 * rr + cc] + W.im[3 * cc + rr]);
-    h[2 * qq] = (T)((double)h[2 * qq] - a.eps * tim);
-    h[2 * qq + 1] = (T)((double)h[2 * qq + 1] + a.eps * tre);
+    hh[2 * qq] = (T)((double)hh[2 * qq] - a.eps * tim);
+    hh[2 * qq + 1] = (T)((double)hh[2 * qq + 1] + a.eps * tre);
   }
-  h[6] = (T)((double)h[6] - a.eps * (W.im[0] - imtr));
-  h[7] = (T)((double)h[7] - a.eps * (W.im[4] - imtr));
+  hh[6] = (T)((double)hh[6] - a.eps * (W.im[0] - imtr));
+  hh[7] = (T)((double)hh[7] - a.eps * (W.im[4] - imtr));
+#pragma unroll
+  for (int k = 0; k < NV; ++k) hv[k] = hr[k];
 }
 
 // The staple-product force, queued on the library stream.  False (nothing
@@ -493,8 +495,6 @@ bool gauge_force_staples(GaugeT<T> *G, ForceT<T> *F, const QPHIX_gauge_coeffs_t 
   GqArgs a{};
   a.u12[0] = G->u12[0];
   a.u12[1] = G->u12[1];
-  a.u[0] = G->u[0];
-  a.u[1] = G->u[1];
   a.q[0] = q;
   a.q[1] = q + per;
   a.h[0] = F->h[0];
