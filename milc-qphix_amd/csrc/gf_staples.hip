@@ -21,10 +21,10 @@
 // with Q_{-a,d}(x) = Q_{a,d}(x-a)^dag.  A rectangle n, n, -mu, -n, -n and the
 // parallelograms with A = n begin with the same link, so each such group is
 // one bracket times L_n(x+mu): 51 SU(3) products per link (with W = U S)
-// instead of the walk kernel's 140, plus 12 per link to form the staples.
+// instead of the walk kernel's 140, plus 10.5 per link to form the staples.
 //
-// Kernel 1 (k_gq_staples): one work-item per (site, mu, nu), both signs of
-// nu; the staples are stored as their rows 0-1 (they are SU(3)), complex-
+// Kernel 1 (k_gq_staples): one work-item per (site, pair mu < nu), the four
+// staples Q_{mu,+-nu}, Q_{nu,+-mu}; the staples are stored as their rows 0-1 (they are SU(3)), complex-
 // major: Q[parity][q = 6 mu + 2 rank(nu) + (n < 0)][element 0..5][site].
 // Kernel 2 (k_gq_force): one work-item per link, as the walk kernel, with the
 // 8 (mu, parity) blocks of a site block next to each other and each XCD
@@ -230,18 +230,25 @@ __device__ __forceinline__ int gq_slot() {
   return nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
 }
 
-// Kernel 1.  One block per (256 sites of one parity, (mu, nu) pair, parity),
-// the 24 (pair, parity) blocks of a site block in adjacent slots, so that
-// the links they share are read once into their XCD's L2.
+// Kernel 1.  One block per (256 sites of one parity, pair mu < nu, parity),
+// the 12 (pair, parity) blocks of a site block in adjacent slots, so that the
+// links they share are read once into their XCD's L2.  A work-item forms the
+// four staples of its pair at its site:
+//   Q_{mu,+nu}(y) = U_nu(y+mu) U_mu(y+nu)^dag U_nu(y)^dag = t U_nu(y)^dag,  t = U_nu(y+mu) U_mu(y+nu)^dag
+//   Q_{nu,+mu}(y) = U_mu(y+nu) U_nu(y+mu)^dag U_mu(y)^dag = t^dag U_mu(y)^dag
+//   Q_{mu,-nu}(y) = U_nu(y+mu-nu)^dag U_mu(y-nu)^dag U_nu(y-nu)
+//   Q_{nu,-mu}(y) = U_mu(y+nu-mu)^dag U_nu(y-mu)^dag U_mu(y-mu)
+// (10 link loads and 7 products for 4 staples: 10.5 products per link).
 template <typename T>
 __global__ void __launch_bounds__(256) k_gq_staples(const T *u0, const T *u1, T *q0, T *q1, GqGeo g) {
   const int slot = gq_slot();
-  const int sb = slot / 24, pp = slot % 24;
+  const int sb = slot / 12, pp = slot % 12;
   const int il = sb * 256 + threadIdx.x;
   if (il >= g.vh) return;
   const int i = gq_site(g, il);
   const int pair = pp >> 1, cb = pp & 1;
-  const int mu = pair / 3, nu = q_other(mu, pair % 3);
+  const int mu = pair < 3 ? 0 : (pair < 5 ? 1 : 2);
+  const int nu = pair < 3 ? pair + 1 : (pair < 5 ? pair - 1 : 3);
   int c[4];
   q_coords(g, i, cb, c);
   const int r = c[0] + g.L[0] * (c[1] + g.L[1] * (c[2] + g.L[2] * c[3]));
@@ -249,25 +256,34 @@ __global__ void __launch_bounds__(256) k_gq_staples(const T *u0, const T *u1, T 
   const int r_p = q_move(g, r, cmu, mu, 1);    // y + mu
   const int r_n = q_move(g, r, cnu, nu, 1);    // y + nu
   const int r_b = q_move(g, r, cnu, nu, -1);   // y - nu
+  const int r_m = q_move(g, r, cmu, mu, -1);   // y - mu
   const int r_pb = q_move(g, r_p, cnu, nu, -1);  // y + mu - nu
+  const int r_nm = q_move(g, r_n, cmu, mu, -1);  // y + nu - mu
   const T *uo = cb ? u0 : u1;  // links of the other parity
   const T *us = cb ? u1 : u0;  // links of this parity
+  T *qs = cb ? q1 : q0;
   M3 a, b, t, w;
-  // Q_{mu,+nu}(y) = U_nu(y+mu) U_mu(y+nu)^dag U_nu(y)^dag
   load2(uo, nu, g.vh, r_p >> 1, a);
   load2(uo, mu, g.vh, r_n >> 1, b);
   m_mul<false, true>(a, b, t);
   load2(us, nu, g.vh, i, a);
   m_mul<false, true>(t, a, w);
-  T *qs = cb ? q1 : q0;
   store2(qs, q_slot(mu, nu, false), g.vh, i, w);
-  // Q_{mu,-nu}(y) = U_nu(y+mu-nu)^dag U_mu(y-nu)^dag U_nu(y-nu)
+  load2(us, mu, g.vh, i, a);
+  m_mul<true, true>(t, a, w);
+  store2(qs, q_slot(nu, mu, false), g.vh, i, w);
   load2(us, nu, g.vh, r_pb >> 1, a);
   load2(uo, mu, g.vh, r_b >> 1, b);
   m_mul<true, true>(a, b, t);
   load2(uo, nu, g.vh, r_b >> 1, a);
   m_mul<false, false>(t, a, w);
   store2(qs, q_slot(mu, nu, true), g.vh, i, w);
+  load2(us, mu, g.vh, r_nm >> 1, a);
+  load2(uo, nu, g.vh, r_m >> 1, b);
+  m_mul<true, true>(a, b, t);
+  load2(uo, mu, g.vh, r_m >> 1, a);
+  m_mul<false, false>(t, a, w);
+  store2(qs, q_slot(nu, mu, true), g.vh, i, w);
 }
 
 struct GqArgs {
@@ -490,7 +506,7 @@ bool gauge_force_staples(GaugeT<T> *G, ForceT<T> *F, const QPHIX_gauge_coeffs_t 
     return e && e[0] == 't';
   }();
   g.zreg = !tenv && gg.nz % 8 == 0 && gg.vh % 256 == 0 && nbx % 8 == 0 ? 1 : 0;
-  hipLaunchKernelGGL(k_gq_staples<T>, dim3((unsigned)(24 * nbx)), dim3(256), 0, ctx().stream, G->u12[0],
+  hipLaunchKernelGGL(k_gq_staples<T>, dim3((unsigned)(12 * nbx)), dim3(256), 0, ctx().stream, G->u12[0],
                      G->u12[1], q, q + per, g);
   GqArgs a{};
   a.u12[0] = G->u12[0];
