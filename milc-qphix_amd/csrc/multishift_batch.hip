@@ -13,7 +13,7 @@
 //    scalars and gate; shifts retire once their zeta-scaled residual meets the
 //    target;
 //  * precision: FP64 vectors and links until every source's |r| is below
-//    QPHIX_MSX_SWITCH (default 1e-3) times |b| -- the shifted systems cannot
+//    QPHIX_MSX_SWITCH (default 3e-4) times |b| -- the shifted systems cannot
 //    be corrected later, and the rounding of the large early steps is what
 //    limits their accuracy -- then FP32 vectors and links, with FP64 solutions
 //    x_j flushed from FP32 accumulators at every reliable update;

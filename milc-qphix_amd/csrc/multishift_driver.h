@@ -379,9 +379,12 @@ static int invert_multi_rhs(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_
   const char *vb = std::getenv("QPHIX_VERBOSE");
   const bool verbose = vb && std::atoi(vb) > 0;
   // |r| / |b| at which the multi-shift iteration drops from FP64 to FP32
-  // (QPHIX_MSX_SWITCH; 0 = FP32 throughout)
+  // (QPHIX_MSX_SWITCH; 0 = FP32 throughout).  Measured at 32^3x64, 8 sources
+  // x 12 shifts: 3.41 / 2.98 / 2.91 / 2.92 / 2.89 s for 3e-3 / 1e-3 / 3e-4 /
+  // 1e-4 / 1e-5 (a later switch leaves the shifted solutions less accurate and
+  // the refinement longer); 48^3x96, 4 x 16 shifts: 7.97 s at 1e-3, 7.74 s at 1e-4.
   const char *sw = std::getenv("QPHIX_MSX_SWITCH");
-  const double switch_frac = sw ? std::atof(sw) : 1e-3;
+  const double switch_frac = sw ? std::atof(sw) : 3e-4;
   for (int pass = 0; pass < ntodo; ++pass) {
     const int parity = todo[pass], other = 1 - parity;
     // 1. multi-shift on blocks of NR sources
