@@ -268,6 +268,15 @@ const LinksT<Q16> *sloppy_links16(LinksT<double> *L);
 // Link bits of the mixed solvers' sloppy operator: 32 (default) or 16.
 int sloppy_link_bits();
 
+struct SolverState;
+// D on output parity cb of n (2..8) vectors src[r] (the other parity's halves)
+// as one multi-RHS Dslash (multishift_batch.hip): the inputs are packed into
+// pin and the results left in pout, both [vh][nr][6] interleaved with nr = 2,
+// 4 or 8 >= n (source r of site i at colour-vector index i * nr + r); st:
+// nr device states used as the sources' gates.  Returns nr.  Queued only.
+template <typename T>
+int mrhs_dslash_packed(const LinksT<T> &L, int cb, const T *const *src, int n, T *pin, T *pout, SolverState *st);
+
 // Mixed-precision CG on one parity system (mixed.hip); out is the initial guess.
 int mixed_solve(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,
                 QPHIX_resid_arg_t *res, double mass, VectorT<double> *out, VectorT<double> *in,

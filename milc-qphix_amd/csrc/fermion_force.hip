@@ -12,7 +12,8 @@
 // and for a Naik link the same with y+3mu (O3).  In PyTorch's convention for
 // complex leaves, dS/dRe U + i dS/dIm U = 2 O^dag.
 //
-// Device work per call: Y_k by the FP64 (FP32) Dslash, then k_fermion_oprod,
+// Device work per call: Y_k by the FP64 (FP32) Dslash (groups of 2..8 terms
+// as one multi-RHS Dslash, Y left interleaved), then k_fermion_oprod,
 // one work-item per (site, mu, parity), accumulating groups of OPROD_GROUP
 // terms in FP64 registers; outputs stay in HBM in a coalesced layout
 // [parity][element][site][mu] (double2) and are exported to the raw MILC
@@ -33,7 +34,9 @@ constexpr int OPROD_GROUP = 8;
 
 template <typename T> struct OprodArgs {
   const T *X[OPROD_GROUP];  // even halves
-  const T *Y[OPROD_GROUP];  // odd halves, D_oe X
+  const T *Y[OPROD_GROUP];  // odd halves, D_oe X (ynr = 0)
+  const T *Yi;              // ynr > 0: the odd halves interleaved, term k of site i at i * ynr + k
+  int ynr;
   double c[OPROD_GROUP];
   int nt;                   // terms in this group (1..OPROD_GROUP)
   int accumulate;           // add to the outputs instead of overwriting them
@@ -113,12 +116,22 @@ __global__ void __launch_bounds__(256) k_fermion_oprod(const OprodArgs<T> a) {
 #pragma unroll
   for (int k = 0; k < OPROD_GROUP; ++k) {
     if (k >= a.nt) break;
-    const T *here = cb ? a.Y[k] : a.X[k];
-    const T *other = cb ? a.X[k] : a.Y[k];
     cplx<T> b[3], f1[3], f3[3];
-    Spin<T>::load(here, i, b);
-    Spin<T>::load(other, n1, f1);
-    Spin<T>::load(other, n3, f3);
+    if (cb) {  // psi(y) = Y, its forward neighbours X
+      if (a.ynr) Spin<T>::load(a.Yi, i * a.ynr + k, b);
+      else Spin<T>::load(a.Y[k], i, b);
+      Spin<T>::load(a.X[k], n1, f1);
+      Spin<T>::load(a.X[k], n3, f3);
+    } else {
+      Spin<T>::load(a.X[k], i, b);
+      if (a.ynr) {
+        Spin<T>::load(a.Yi, n1 * a.ynr + k, f1);
+        Spin<T>::load(a.Yi, n3 * a.ynr + k, f3);
+      } else {
+        Spin<T>::load(a.Y[k], n1, f1);
+        Spin<T>::load(a.Y[k], n3, f3);
+      }
+    }
     const double c = eps * a.c[k];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -174,8 +187,11 @@ static bool oprod_device(const LinksT<T> &L, int nterms, const double *coef,
   const Geometry &gg = ctx().g;
   const int vh = gg.vh;
   const size_t nv = 6 * (size_t)vh;
-  T *ybuf = (T *)ws2_get(OPROD_GROUP * nv * sizeof(T));
+  // packed inputs and outputs of the multi-RHS Dslash, then its source gates
+  T *ybuf = (T *)ws2_get(2 * OPROD_GROUP * nv * sizeof(T) + OPROD_GROUP * sizeof(SolverState) + 256);
   if (!ybuf) return false;
+  T *ypack = ybuf + OPROD_GROUP * nv;
+  SolverState *yst = (SolverState *)(((uintptr_t)(ypack + OPROD_GROUP * nv) + 255) & ~(uintptr_t)255);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (kernel_ms) {
     *kernel_ms = 0.f;
@@ -190,12 +206,16 @@ static bool oprod_device(const LinksT<T> &L, int nterms, const double *coef,
     const int nt = std::min(OPROD_GROUP, nterms - k0);
     OprodArgs<T> a{};
     for (int k = 0; k < nt; ++k) {
-      T *y = ybuf + (size_t)k * nv;
-      DslashExtra ex;
-      launch_dslash<T>(L, 1, y, X[k0 + k]->half[0], EPI_PLAIN, ex);
       a.X[k] = X[k0 + k]->half[0];
-      a.Y[k] = y;
       a.c[k] = coef[k0 + k];
+    }
+    if (nt == 1) {  // Y = D_oe X
+      DslashExtra ex;
+      launch_dslash<T>(L, 1, ybuf, a.X[0], EPI_PLAIN, ex);
+      a.Y[0] = ybuf;
+    } else {  // the group's Y_k as one multi-RHS Dslash: links read once for all terms
+      a.ynr = mrhs_dslash_packed<T>(L, 1, a.X, nt, ypack, ybuf, yst);
+      a.Yi = ybuf;
     }
     a.nt = nt;
     a.accumulate = k0 > 0;

@@ -532,6 +532,50 @@ __global__ void k_mrhs_unpack(float *const *dst, const float *src, int vh, int n
   Spin<float>::store(dst[r], i, v);
 }
 
+// Sources passed by value (no device pointer array to stage).
+template <typename T> struct PackSrc {
+  const T *p[8];
+};
+
+template <typename T>
+__global__ void k_mrhs_pack_args(T *dst, PackSrc<T> src, int vh, int n, int NR) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= vh * NR) return;
+  const int i = t / NR, r = t % NR;
+  cplx<T> v[3] = {{0, 0}, {0, 0}, {0, 0}};
+  if (r < n) {
+    const T *s = src.p[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) s = r == k ? src.p[k] : s;  // selects, no indexed private array
+    Spin<T>::load(s, i, v);
+  }
+  Spin<T>::store(dst, t, v);
+}
+
+__global__ void k_mrhs_gates(SolverState *st, int n, int nr) {
+  const int r = threadIdx.x;
+  if (r < nr) st[r].active = r < n ? 1 : 0;
+}
+
+template <typename T>
+int mrhs_dslash_packed(const LinksT<T> &L, int cb, const T *const *src, int n, T *pin, T *pout, SolverState *st) {
+  const int vh = ctx().g.vh;
+  const int nr = n <= 2 ? 2 : (n <= 4 ? 4 : 8);
+  PackSrc<T> ps{};
+  for (int r = 0; r < n; ++r) ps.p[r] = src[r];
+  hipLaunchKernelGGL(k_mrhs_gates, dim3(1), dim3(64), 0, ctx().stream, st, n, nr);
+  hipLaunchKernelGGL(k_mrhs_pack_args<T>, dim3((vh * nr + BLAS_BLOCK - 1) / BLAS_BLOCK), dim3(BLAS_BLOCK), 0,
+                     ctx().stream, pin, ps, vh, n, nr);
+  if (nr == 2) mrhs_dslash<2, T>(L, cb, pout, pin, EPI_PLAIN, (T)0, nullptr, st, 1);
+  else if (nr == 4) mrhs_dslash<4, T>(L, cb, pout, pin, EPI_PLAIN, (T)0, nullptr, st, 1);
+  else mrhs_dslash<8, T>(L, cb, pout, pin, EPI_PLAIN, (T)0, nullptr, st, 1);
+  return nr;
+}
+template int mrhs_dslash_packed<double>(const LinksT<double> &, int, const double *const *, int, double *, double *,
+                                        SolverState *);
+template int mrhs_dslash_packed<float>(const LinksT<float> &, int, const float *const *, int, float *, float *,
+                                       SolverState *);
+
 template <int NR>
 static double dslash_multi(LinksT<float> *L, QPHIX_F3_ColorVector **out, QPHIX_F3_ColorVector **in,
                            int n, int cb, int reps, int K) {

@@ -66,6 +66,31 @@ def test_oprod_matches_oracle(prec):
         o.free()
 
 
+@pytest.mark.parametrize("nterms", [2, 3, 5])
+def test_oprod_term_groups(nterms):
+    """Groups of 2..8 terms get their D_oe X_k from one multi-RHS Dslash with
+    2, 4 or 8 interleaved sources (3 and 5 terms leave padding sources
+    unused): the outer products against the oracle."""
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    fat, lng = F.make_links(lat, "weak", 4243, eps=0.3)
+    raw = F.raw_link_set(lat, fat, lng)
+    L = A.Links(s, *raw)
+    xs = _solutions(s, lat, L, [0.07, 0.3], (4, 5, 6))[:nterms]
+    coef = [0.4, -0.9, 0.7, 1.2, -0.3][:nterms]
+    vx = [A.Vector(s, x, A.EVEN) for x in xs]
+    print(f"[case] oprod D 8^4 {nterms} terms", flush=True)
+    o1, o3, info = A.fermion_force_oprod(L, coef, vx)
+    orc = O.Oracle(dims, *raw)
+    r1, r3 = orc.fermion_oprod(coef, xs)
+    orc.close()
+    assert rel_l2(o1, r1.view(np.complex128).reshape(o1.shape)) < 1e-12
+    assert rel_l2(o3, r3.view(np.complex128).reshape(o3.shape)) < 1e-12
+    for o in vx + [L]:
+        o.free()
+
+
 def test_oprod_edge_cases():
     dims = (8, 8, 8, 8)
     s = lattice_session(dims)
