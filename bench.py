@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the staggered hot path on one MI355X.
 
-Headline (BASELINE metric): HISQ Dslash GB/s and fraction of HBM peak on a 32^4
+Headline metric: HISQ Dslash GB/s and fraction of HBM peak on a 32^4
 lattice, plus FP64 vs mixed-precision CG time to a 1e-10 true residual.
 
 A "step" is one single-parity even-odd Dslash over the 32^4 lattice's 524288

@@ -17,7 +17,7 @@ from pyqphix import fields as F
 pytestmark = pytest.mark.gpu
 NP = {"D": np.float64, "F": np.float32}
 
-# 8^4, fat = 1 + 0.1 Gaussian, long = Haar SU(3), m = 0.05 in
+# The 8^4 CPU-runnable case: fat = 1 + 0.1 Gaussian, long = Haar SU(3), m = 0.05 in
 # M = m^2 - D_eo D_oe, i.e. API mass m/2 (the API solves 4 mass^2 - D D);
 # CG to 1e-8 relative residual = resid 1e-16 (squared).
 CFG1 = dict(dims=(8, 8, 8, 8), kind="haar", seed=1001, mass=0.025, resid=1e-16)
