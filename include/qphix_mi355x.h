@@ -79,7 +79,7 @@ void QPHIX_MI355X_set_fewsums(int mode);
 void QPHIX_MI355X_set_mrhs_lds(int on);
 /* The batched multi-shift solver updates the shifted systems' x_j and p_j
  * `steps` steps at a time (each read and written once per group of steps; the
- * base system every step): 4 (the default and the largest), 3, 2, or 1 (every
+ * base system every step): 8 (the default and the largest) down to 1 (every
  * step; 0 and negative values mean 1).  All give identical results. */
 void QPHIX_MI355X_set_msx_fuse(int steps);
 /* Links of the mixed solvers' sloppy operator (QPHIX_D3_asqtad_invert_mixed):

@@ -12,7 +12,7 @@
 namespace qphix {
 
 constexpr int MAXNDMASS = 150;  // ks_d_congrad_fn.h:4
-constexpr int MSX_MAXDEFER = 4;  // batched multi-shift: most steps per shifted update
+constexpr int MSX_MAXDEFER = 8;  // batched multi-shift: most steps per shifted update
 
 struct SolverState {
   double rsq, oldrsq, pkp, relrsq, src_norm;

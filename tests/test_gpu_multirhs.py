@@ -167,7 +167,7 @@ def test_lds_staged_links_match_per_lane_loads():
         o.free()
 
 
-@pytest.mark.parametrize("block,steps", [(8, 4), (4, 4), (8, 3), (8, 2)])
+@pytest.mark.parametrize("block,steps", [(8, 8), (4, 8), (8, 6), (8, 4), (4, 4), (8, 3), (8, 2)])
 def test_deferred_shifted_updates_match_single_steps(block, steps):
     """The batched solver's shifted systems updated `steps` steps at a time
     (one read and write of x_j, p_j per group of steps) give the same solutions,
@@ -198,7 +198,7 @@ def test_deferred_shifted_updates_match_single_steps(block, steps):
                 for x in row:
                     x.free()
     finally:
-        s.lib.QPHIX_MI355X_set_msx_fuse(4)  # the default
+        s.lib.QPHIX_MI355X_set_msx_fuse(8)  # the default
     assert got[steps][0] == got[1][0]
     assert got[steps][2] == got[1][2]
     for r1, r0 in zip(got[steps][1], got[1][1]):
