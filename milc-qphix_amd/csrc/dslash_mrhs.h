@@ -361,4 +361,165 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds(const MrhsArgs
   }
 }
 
+// Sources r0 and r0 + 1 of interleaved index j = site * NR + r0 (r0 even):
+// neighbours in the tile-major layout, so for FP32 each colour of both is one
+// 16-byte load.
+template <typename T> struct Spin2 {
+  static __device__ __forceinline__ void load(const T *p, int j, cplx<T> *a, cplx<T> *b) {
+    Spin<T>::load(p, j, a);
+    Spin<T>::load(p, j + 1, b);
+  }
+};
+template <> struct Spin2<float> {
+  static __device__ __forceinline__ void load(const float *p, int j, cplx<float> *a, cplx<float> *b) {
+    const float2 *q = reinterpret_cast<const float2 *>(p) + spin_cidx((size_t)j, 0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float4 d = *reinterpret_cast<const float4 *>(q + SPIN_TILE * c);
+      a[c] = cplx<float>{d.x, d.y};
+      b[c] = cplx<float>{d.z, d.w};
+    }
+  }
+};
+
+// Two sources per work-item (K = 2) with the links staged through LDS as in
+// k_dslash_mrhs_lds: NR / 2 lanes per site, SW = 128 / NR sites per
+// wavefront.  Every link element a lane reads from LDS serves both its
+// sources, and a lane's two sources are neighbours in the interleaved layout
+// (FP32: one 16-byte load per gathered colour for both).  Per source the
+// arithmetic and its order are those of k_dslash_mrhs<T, NL, NR, 2, EPI>.
+template <typename T, int NL, int NR, int EPI>
+__global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds2(const MrhsArgsT<T> a) {
+  static_assert(NR >= 4 && 128 % NR == 0, "NR / 2 lanes per site, whole sites per wavefront");
+  using V = typename vec16<T>::type;
+  constexpr int CPV = vec16<T>::ncplx;
+  constexpr int G = NR / 2;                                               // lanes per site
+  constexpr int NCH_F = 18 / CPV, NCH_L = NL / CPV, NCH = NCH_F + NCH_L;  // chunks per site and direction
+  constexpr int SW = 64 / G;                                              // sites per wavefront
+  constexpr int CW = SW * NCH;                                            // chunks per wavefront and direction
+  constexpr int QL = (CW + 63) / 64;                                      // chunks loaded per lane
+  __shared__ V lds[DSLASH_BLOCK / 64][CW];
+  const unsigned mask = rhs_mask<NR>(a.states);
+  if (mask == 0) return;  // uniform over the grid
+  const int vh = a.g.vh;
+  const int lane = threadIdx.x & 63;
+  V *wl = lds[threadIdx.x >> 6];
+  const XcdWalk wk(a.nlog);
+  double v[2] = {0.0, 0.0};
+  for (int step = 0;; ++step) {
+    const int blk = wk.at(step);
+    if (blk < 0) break;
+    const int L0 = (blk * DSLASH_BLOCK + (threadIdx.x & ~63)) / G;  // first site of the wavefront
+    if (L0 >= vh) continue;                                          // uniform per wavefront
+    int lsite[QL], lk[QL];
+#pragma unroll
+    for (int q = 0; q < QL; ++q) {
+      const int c = lane + 64 * q, k = c / SW, Ls = L0 + c % SW;
+      lk[q] = k;
+      lsite[q] = (c < CW && Ls < vh) ? ordered_site(a.g, a.order, Ls) : -1;
+    }
+    auto fetch = [&](int mu, V *pre) {
+#pragma unroll
+      for (int q = 0; q < QL; ++q) {
+        const int i = lsite[q], k = lk[q];
+        if (i < 0) continue;
+        pre[q] = k < NCH_F ? nt_chunk(a.fat + link_chunk(mu, k, i, NCH_F))
+                           : nt_chunk(a.lng + link_chunk(mu, k - NCH_F, i, NCH_L));
+      }
+    };
+    // this lane's site and sources r0, r0 + 1
+    const int s = lane / G, r0 = (lane % G) * 2, L = L0 + s;
+    const unsigned m = (mask >> r0) & 3u;
+    const bool active = L < vh && m != 0;
+    int i = 0;
+    SiteNbr n{};
+    unsigned sg = 0;
+    if (active) {
+      i = ordered_site(a.g, a.order, L);
+      site_neighbours(a.g, i, a.cb, n);
+      if constexpr (NL == 12) sg = a.lsign[i];
+    }
+    cplx<T> acc[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[k][c] = cplx<T>{0, 0};
+    V pre[QL];
+    fetch(0, pre);
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      wave_lds_sync();  // the previous direction's reads are done
+#pragma unroll
+      for (int q = 0; q < QL; ++q)
+        if (lsite[q] >= 0) wl[lk[q] * SW + (lane + 64 * q) % SW] = pre[q];
+      wave_lds_sync();
+      if (mu < 3) fetch(mu + 1, pre);
+      if (active) {
+        auto lds_c = [&](int c0, int k) -> cplx<T> {
+          const V d = wl[(c0 + k / CPV) * SW + s];
+          if constexpr (CPV == 2) return (k % 2) ? cplx<T>{d.z, d.w} : cplx<T>{d.x, d.y};
+          else return cplx<T>{d.x, d.y};
+        };
+        // one link and one neighbour at a time, applied to both sources
+        auto apply = [&](const cplx<T> *w, int nb) {
+          cplx<T> v0[3], v1[3];
+          Spin2<T>::load(a.in, nb * NR + r0, v0, v1);
+          matvec_acc(w, v0, acc[0]);
+          matvec_acc(w, v1, acc[1]);
+        };
+        cplx<T> w[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) w[e] = lds_c(0, e);
+        apply(w, n.f1[mu]);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) w[e] = lds_c(0, 9 + e);
+        apply(w, n.b1[mu]);
+        if constexpr (NL == 18) {
+#pragma unroll
+          for (int e = 0; e < 9; ++e) w[e] = lds_c(NCH_F, e);
+          apply(w, n.f3[mu]);
+#pragma unroll
+          for (int e = 0; e < 9; ++e) w[e] = lds_c(NCH_F, 9 + e);
+          apply(w, n.b3[mu]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 6; ++e) w[e] = lds_c(NCH_F, e);
+          recon_row2(w, ((sg >> (2 * mu + 1)) & 1u) ? (T)-1 : (T)1);
+          apply(w, n.f3[mu]);
+#pragma unroll
+          for (int e = 0; e < 6; ++e) w[e] = lds_c(NCH_F, 6 + e);
+          recon_row2(w, ((sg >> (2 * mu)) & 1u) ? (T)-1 : (T)1);
+          apply(w, n.b3[mu]);
+        }
+      }
+      // the next direction's gathers stay after this one's: hoisted, the
+      // loads of all four directions (two sources each) are live together
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (!((m >> k) & 1u)) continue;
+        const int slot = i * NR + r0 + k;
+        if constexpr (EPI == EPI_PKP) {
+          cplx<T> p[3];
+          Spin<T>::load(a.aux, slot, p);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            acc[k][c].re = fma(a.s, p[c].re, acc[k][c].re);
+            acc[k][c].im = fma(a.s, p[c].im, acc[k][c].im);
+            v[k] = fma((double)p[c].re, (double)acc[k][c].re, v[k]);
+            v[k] = fma((double)p[c].im, (double)acc[k][c].im, v[k]);
+          }
+        }
+        Spin<T>::store(a.out, slot, acc[k]);
+      }
+    }
+  }
+  if constexpr (EPI == EPI_PKP) {
+    const double sum = group_block_sum<G, 2, DSLASH_BLOCK>(v);
+    if (threadIdx.x < NR) a.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sum;
+  }
+}
+
 } // namespace qphix

@@ -117,11 +117,14 @@ def test_multi_rhs_graph_replay_matches_plain_launches():
         o.free()
 
 
-def test_lds_staged_links_match_per_lane_loads():
+@pytest.mark.parametrize("k", [1, 2])
+def test_lds_staged_links_match_per_lane_loads(k, monkeypatch):
     """The multi-RHS Dslash with links staged through LDS and the one that
     loads them per work-item give bit-identical results: plain Dslash on 8 and
     4 FP32 vectors, and a batched multi-shift solve (FP64 and FP32 phases, the
-    p.Ap epilogue, refinement)."""
+    p.Ap epilogue, refinement).  k: sources per work-item (2: the LDS kernel
+    that applies each staged link to two sources)."""
+    monkeypatch.setenv("QPHIX_MRHS_K", str(k))
     dims = (8, 8, 8, 8)
     s = lattice_session(dims)
     lat = F.Lattice(dims)
@@ -137,7 +140,7 @@ def test_lds_staged_links_match_per_lane_loads():
     try:
         for on in (1, 0):
             s.lib.QPHIX_MI355X_set_mrhs_lds(on)
-            print(f"[case] mrhs lds={on}", flush=True)
+            print(f"[case] mrhs lds={on} k={k}", flush=True)
             outs = []
             for n in (8, 4):
                 vout = [A.Vector(s, None, A.EVENODD, "F") for _ in range(n)]
