@@ -361,39 +361,42 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds(const MrhsArgs
   }
 }
 
-// Sources r0 and r0 + 1 of interleaved index j = site * NR + r0 (r0 even):
-// neighbours in the tile-major layout, so for FP32 each colour of both is one
-// 16-byte load.
-template <typename T> struct Spin2 {
-  static __device__ __forceinline__ void load(const T *p, int j, cplx<T> *a, cplx<T> *b) {
-    Spin<T>::load(p, j, a);
-    Spin<T>::load(p, j + 1, b);
+// Sources r0 .. r0 + K - 1 of interleaved index j = site * NR + r0 (r0 a
+// multiple of K, K dividing 64): neighbours in one tile of the tile-major
+// layout, so for FP32 each colour of two sources is one 16-byte load.
+template <typename T, int K> struct SpinK {
+  static __device__ __forceinline__ void load(const T *p, int j, cplx<T> (*v)[3]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) Spin<T>::load(p, j + k, v[k]);
   }
 };
-template <> struct Spin2<float> {
-  static __device__ __forceinline__ void load(const float *p, int j, cplx<float> *a, cplx<float> *b) {
+template <int K> struct SpinK<float, K> {
+  static_assert(K % 2 == 0, "pairs of sources");
+  static __device__ __forceinline__ void load(const float *p, int j, cplx<float> (*v)[3]) {
     const float2 *q = reinterpret_cast<const float2 *>(p) + spin_cidx((size_t)j, 0);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float4 d = *reinterpret_cast<const float4 *>(q + SPIN_TILE * c);
-      a[c] = cplx<float>{d.x, d.y};
-      b[c] = cplx<float>{d.z, d.w};
-    }
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int h = 0; h < K / 2; ++h) {
+        const float4 d = reinterpret_cast<const float4 *>(q + SPIN_TILE * c)[h];
+        v[2 * h][c] = cplx<float>{d.x, d.y};
+        v[2 * h + 1][c] = cplx<float>{d.z, d.w};
+      }
   }
 };
 
-// Two sources per work-item (K = 2) with the links staged through LDS as in
-// k_dslash_mrhs_lds: NR / 2 lanes per site, SW = 128 / NR sites per
-// wavefront.  Every link element a lane reads from LDS serves both its
-// sources, and a lane's two sources are neighbours in the interleaved layout
-// (FP32: one 16-byte load per gathered colour for both).  Per source the
-// arithmetic and its order are those of k_dslash_mrhs<T, NL, NR, 2, EPI>.
-template <typename T, int NL, int NR, int EPI>
-__global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds2(const MrhsArgsT<T> a) {
-  static_assert(NR >= 4 && 128 % NR == 0, "NR / 2 lanes per site, whole sites per wavefront");
+// K = 2 or 4 sources per work-item with the links staged through LDS as in
+// k_dslash_mrhs_lds: G = NR / K lanes per site, SW = 64 / G sites per
+// wavefront.  Every link element a lane reads from LDS serves its K sources,
+// and a lane's sources are neighbours in the interleaved layout (FP32: one
+// 16-byte load per gathered colour for each pair).  Per source the arithmetic
+// and its order are those of k_dslash_mrhs<T, NL, NR, K, EPI>.
+template <typename T, int NL, int NR, int K, int EPI>
+__global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_ldsk(const MrhsArgsT<T> a) {
+  static_assert((K == 2 || K == 4) && NR >= 2 * K && 64 % (NR / K) == 0, "G >= 2 lanes per site");
   using V = typename vec16<T>::type;
   constexpr int CPV = vec16<T>::ncplx;
-  constexpr int G = NR / 2;                                               // lanes per site
+  constexpr int G = NR / K;                                               // lanes per site
   constexpr int NCH_F = 18 / CPV, NCH_L = NL / CPV, NCH = NCH_F + NCH_L;  // chunks per site and direction
   constexpr int SW = 64 / G;                                              // sites per wavefront
   constexpr int CW = SW * NCH;                                            // chunks per wavefront and direction
@@ -405,7 +408,9 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds2(const MrhsArg
   const int lane = threadIdx.x & 63;
   V *wl = lds[threadIdx.x >> 6];
   const XcdWalk wk(a.nlog);
-  double v[2] = {0.0, 0.0};
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
   for (int step = 0;; ++step) {
     const int blk = wk.at(step);
     if (blk < 0) break;
@@ -427,9 +432,9 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds2(const MrhsArg
                            : nt_chunk(a.lng + link_chunk(mu, k - NCH_F, i, NCH_L));
       }
     };
-    // this lane's site and sources r0, r0 + 1
-    const int s = lane / G, r0 = (lane % G) * 2, L = L0 + s;
-    const unsigned m = (mask >> r0) & 3u;
+    // this lane's site and sources r0 .. r0 + K - 1
+    const int s = lane / G, r0 = (lane % G) * K, L = L0 + s;
+    const unsigned m = (mask >> r0) & ((1u << K) - 1u);
     const bool active = L < vh && m != 0;
     int i = 0;
     SiteNbr n{};
@@ -439,9 +444,9 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds2(const MrhsArg
       site_neighbours(a.g, i, a.cb, n);
       if constexpr (NL == 12) sg = a.lsign[i];
     }
-    cplx<T> acc[2][3];
+    cplx<T> acc[K][3];
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < K; ++k)
 #pragma unroll
       for (int c = 0; c < 3; ++c) acc[k][c] = cplx<T>{0, 0};
     V pre[QL];
@@ -460,12 +465,12 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds2(const MrhsArg
           if constexpr (CPV == 2) return (k % 2) ? cplx<T>{d.z, d.w} : cplx<T>{d.x, d.y};
           else return cplx<T>{d.x, d.y};
         };
-        // one link and one neighbour at a time, applied to both sources
+        // one link and one neighbour at a time, applied to the K sources
         auto apply = [&](const cplx<T> *w, int nb) {
-          cplx<T> v0[3], v1[3];
-          Spin2<T>::load(a.in, nb * NR + r0, v0, v1);
-          matvec_acc(w, v0, acc[0]);
-          matvec_acc(w, v1, acc[1]);
+          cplx<T> vk[K][3];
+          SpinK<T, K>::load(a.in, nb * NR + r0, vk);
+#pragma unroll
+          for (int k = 0; k < K; ++k) matvec_acc(w, vk[k], acc[k]);
         };
         cplx<T> w[9];
 #pragma unroll
@@ -493,12 +498,12 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds2(const MrhsArg
         }
       }
       // the next direction's gathers stay after this one's: hoisted, the
-      // loads of all four directions (two sources each) are live together
+      // loads of all four directions (K sources each) are live together
       __builtin_amdgcn_sched_barrier(0);
     }
     if (active) {
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
+      for (int k = 0; k < K; ++k) {
         if (!((m >> k) & 1u)) continue;
         const int slot = i * NR + r0 + k;
         if constexpr (EPI == EPI_PKP) {
@@ -517,7 +522,7 @@ __global__ void __launch_bounds__(DSLASH_BLOCK) k_dslash_mrhs_lds2(const MrhsArg
     }
   }
   if constexpr (EPI == EPI_PKP) {
-    const double sum = group_block_sum<G, 2, DSLASH_BLOCK>(v);
+    const double sum = group_block_sum<G, K, DSLASH_BLOCK>(v);
     if (threadIdx.x < NR) a.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sum;
   }
 }

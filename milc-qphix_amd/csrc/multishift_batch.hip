@@ -379,8 +379,8 @@ static bool mrhs_lds_on() {
 }
 
 // K sources per work-item with links staged through LDS: K = 1 (NR >= 2,
-// k_dslash_mrhs_lds) and K = 2 (NR >= 4, k_dslash_mrhs_lds2).
-template <int NR, int K> constexpr bool mrhs_has_lds() { return (K == 1 && NR >= 2) || (K == 2 && NR >= 4); }
+// k_dslash_mrhs_lds) and K = 2, 4 (NR >= 2 K, k_dslash_mrhs_ldsk).
+template <int NR, int K> constexpr bool mrhs_has_lds() { return (K == 1 && NR >= 2) || (K >= 2 && NR >= 2 * K); }
 
 // Resident grid of the kernel a launch uses (cached per kernel; the lattice is
 // fixed between QPHIX_init and QPHIX_finalize).
@@ -392,8 +392,8 @@ template <typename T, int NR, int K, int NL, int EPI> static int mrhs_grid_of(in
     if constexpr (K == 1 && NR >= 2) {
       if (v) g[v] = resident_grid(k_dslash_mrhs_lds<T, NL, NR, EPI>, mrhs_nlog(vh, NR, K));
       else g[v] = resident_grid(k_dslash_mrhs<T, NL, NR, K, EPI>, mrhs_nlog(vh, NR, K));
-    } else if constexpr (K == 2 && NR >= 4) {
-      if (v) g[v] = resident_grid(k_dslash_mrhs_lds2<T, NL, NR, EPI>, mrhs_nlog(vh, NR, K));
+    } else if constexpr (K >= 2 && NR >= 2 * K) {
+      if (v) g[v] = resident_grid(k_dslash_mrhs_ldsk<T, NL, NR, K, EPI>, mrhs_nlog(vh, NR, K));
       else g[v] = resident_grid(k_dslash_mrhs<T, NL, NR, K, EPI>, mrhs_nlog(vh, NR, K));
     } else {
       g[v] = resident_grid(k_dslash_mrhs<T, NL, NR, K, EPI>, mrhs_nlog(vh, NR, K));
@@ -420,22 +420,22 @@ template <typename T, int NR, int K>
 static void mrhs_launch(const LinksT<T> &L, int epi, MrhsArgsT<T> a) {
   const int vh = a.g.vh;
   a.nlog = mrhs_nlog(vh, NR, K);
-  if constexpr (K == 2 && NR >= 4) {
+  if constexpr (K >= 2 && NR >= 2 * K) {
     if (mrhs_lds_on()) {
       const dim3 blk(DSLASH_BLOCK);
       if (L.recon == 12) {
         if (epi == EPI_PLAIN)
-          hipLaunchKernelGGL((k_dslash_mrhs_lds2<T, 12, NR, EPI_PLAIN>), dim3(mrhs_grid_of<T, NR, K, 12, EPI_PLAIN>(vh)),
+          hipLaunchKernelGGL((k_dslash_mrhs_ldsk<T, 12, NR, K, EPI_PLAIN>), dim3(mrhs_grid_of<T, NR, K, 12, EPI_PLAIN>(vh)),
                              blk, 0, ctx().stream, a);
         else
-          hipLaunchKernelGGL((k_dslash_mrhs_lds2<T, 12, NR, EPI_PKP>), dim3(mrhs_grid_of<T, NR, K, 12, EPI_PKP>(vh)),
+          hipLaunchKernelGGL((k_dslash_mrhs_ldsk<T, 12, NR, K, EPI_PKP>), dim3(mrhs_grid_of<T, NR, K, 12, EPI_PKP>(vh)),
                              blk, 0, ctx().stream, a);
       } else {
         if (epi == EPI_PLAIN)
-          hipLaunchKernelGGL((k_dslash_mrhs_lds2<T, 18, NR, EPI_PLAIN>), dim3(mrhs_grid_of<T, NR, K, 18, EPI_PLAIN>(vh)),
+          hipLaunchKernelGGL((k_dslash_mrhs_ldsk<T, 18, NR, K, EPI_PLAIN>), dim3(mrhs_grid_of<T, NR, K, 18, EPI_PLAIN>(vh)),
                              blk, 0, ctx().stream, a);
         else
-          hipLaunchKernelGGL((k_dslash_mrhs_lds2<T, 18, NR, EPI_PKP>), dim3(mrhs_grid_of<T, NR, K, 18, EPI_PKP>(vh)),
+          hipLaunchKernelGGL((k_dslash_mrhs_ldsk<T, 18, NR, K, EPI_PKP>), dim3(mrhs_grid_of<T, NR, K, 18, EPI_PKP>(vh)),
                              blk, 0, ctx().stream, a);
       }
       return;

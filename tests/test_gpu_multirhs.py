@@ -117,13 +117,14 @@ def test_multi_rhs_graph_replay_matches_plain_launches():
         o.free()
 
 
-@pytest.mark.parametrize("k", [1, 2])
+@pytest.mark.parametrize("k", [1, 2, 4])
 def test_lds_staged_links_match_per_lane_loads(k, monkeypatch):
     """The multi-RHS Dslash with links staged through LDS and the one that
     loads them per work-item give bit-identical results: plain Dslash on 8 and
     4 FP32 vectors, and a batched multi-shift solve (FP64 and FP32 phases, the
-    p.Ap epilogue, refinement).  k: sources per work-item (2: the LDS kernel
-    that applies each staged link to two sources)."""
+    p.Ap epilogue, refinement).  k: sources per work-item (2, 4: the LDS
+    kernel that applies each staged link to k sources; the FP64 phase uses at
+    most 2)."""
     monkeypatch.setenv("QPHIX_MRHS_K", str(k))
     dims = (8, 8, 8, 8)
     s = lattice_session(dims)
