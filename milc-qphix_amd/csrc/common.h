@@ -276,6 +276,9 @@ struct SolverState;
 // nr device states used as the sources' gates.  Returns nr.  Queued only.
 template <typename T>
 int mrhs_dslash_packed(const LinksT<T> &L, int cb, const T *const *src, int n, T *pin, T *pout, SolverState *st);
+// The first n sources of an interleaved [vh][nr][6] buffer into n vectors
+// dst + r * 6 vh (multishift_batch.hip).  Queued only.
+template <typename T> void mrhs_unpack(const T *src, T *dst, int n, int nr);
 
 // Mixed-precision CG on one parity system (mixed.hip); out is the initial guess.
 int mixed_solve(QPHIX_info_t *info, LinksT<double> *L, QPHIX_invert_arg_t *inv,

@@ -599,6 +599,26 @@ int mrhs_dslash_packed(const LinksT<T> &L, int cb, const T *const *src, int n, T
   else mrhs_dslash<8, T>(L, cb, pout, pin, EPI_PLAIN, (T)0, nullptr, st, 1);
   return nr;
 }
+template <typename T>
+__global__ void k_mrhs_unpack_strided(T *dst, size_t nv, const T *src, int vh, int n, int NR) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= vh * NR) return;
+  const int i = t / NR, r = t % NR;
+  if (r >= n) return;
+  cplx<T> v[3];
+  Spin<T>::load(src, t, v);
+  Spin<T>::store(dst + (size_t)r * nv, i, v);
+}
+
+template <typename T> void mrhs_unpack(const T *src, T *dst, int n, int nr) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_mrhs_unpack_strided<T>, dim3((vh * nr + BLAS_BLOCK - 1) / BLAS_BLOCK), dim3(BLAS_BLOCK), 0,
+                     ctx().stream, dst, 6 * (size_t)vh, src, vh, n, nr);
+  HIP_CHECK(hipGetLastError());
+}
+template void mrhs_unpack<double>(const double *, double *, int, int);
+template void mrhs_unpack<float>(const float *, float *, int, int);
+
 template int mrhs_dslash_packed<double>(const LinksT<double> &, int, const double *const *, int, double *, double *,
                                         SolverState *);
 template int mrhs_dslash_packed<float>(const LinksT<float> &, int, const float *const *, int, float *, float *,
