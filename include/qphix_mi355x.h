@@ -72,10 +72,10 @@ void QPHIX_MI355X_debug_fail_alloc(int after);
  * environment sets the initial value). */
 void QPHIX_MI355X_set_fewsums(int mode);
 /* on != 0 (default; QPHIX_MRHS_LDS=0 in the environment sets the initial
- * value to 0): multi-RHS Dslash launches with one source per work-item stage
- * each wavefront's links through LDS (loaded once per site instead of once
- * per source); on == 0: every work-item loads its own links.  Both give
- * identical results. */
+ * value to 0): multi-RHS Dslash launches with one, two or four sources per
+ * work-item (the autotuner's choice) stage each wavefront's links through LDS
+ * (loaded once per site instead of once per work-item); on == 0: every
+ * work-item loads its own links.  Both give identical results. */
 void QPHIX_MI355X_set_mrhs_lds(int on);
 /* The batched multi-shift solver updates the shifted systems' x_j and p_j
  * `steps` steps at a time (each read and written once per group of steps; the
