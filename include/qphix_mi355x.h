@@ -77,7 +77,8 @@ void QPHIX_MI355X_set_fewsums(int mode);
  * (loaded once per site instead of once per work-item); on == 0: every
  * work-item loads its own links.  Both give identical results. */
 void QPHIX_MI355X_set_mrhs_lds(int on);
-/* The batched multi-shift solver updates the shifted systems' x_j and p_j
+/* Both multi-shift solvers (QPHIX_*3_asqtad_invert_multi and the batched
+ * QPHIX_D3_asqtad_invert_multi_rhs) update the shifted systems' x_j and p_j
  * `steps` steps at a time (each read and written once per group of steps; the
  * base system every step): 8 (the default and the largest) down to 1 (every
  * step; 0 and negative values mean 1).  All give identical results. */

@@ -44,6 +44,11 @@ void blas_ms_update_pm(T *const *x, T *const *p, const T *r, int nshift, const S
                        const int *gate);
 template <typename T>
 void blas_ms_update_dest(T *const *x, T *const *p, int nshift, const SolverState *st);
+// update_pm with the shifted systems up to st->fuse steps at a time (rprev:
+// fuse - 1 saved residuals); flush = true applies what is still pending.
+template <typename T>
+void blas_ms_update_pm_def(T *const *x, T *const *p, const T *r, T *rprev, int nshift, SolverState *st,
+                           const int *gate, bool flush);
 void set_int(int *p, int v);
 
 } // namespace qphix

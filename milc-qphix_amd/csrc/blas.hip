@@ -258,6 +258,21 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   grid_reduce<1>(s, rc);
 }
 
+// One step of shift j: x_j += beta_j p_j ; p_j = zeta_j r + alpha_j p_j, with
+// zeta_j r formed in FP64 and rounded to the vector type (explicit fmas: every
+// kernel below rounds identically).
+template <typename C>
+__device__ __forceinline__ void ms_one(cplx<C> *xv, cplx<C> *pv, const cplx<C> *rv, C bj, C aj, double zj) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    xv[c].re = fma(bj, pv[c].re, xv[c].re);
+    xv[c].im = fma(bj, pv[c].im, xv[c].im);
+    const C tr = (C)((double)rv[c].re * zj), ti = (C)((double)rv[c].im * zj);
+    pv[c].re = fma(aj, pv[c].re, tr);
+    pv[c].im = fma(aj, pv[c].im, ti);
+  }
+}
+
 // multi-shift update_pm: x_j += beta_j p_j ; p_j = zeta_j r + alpha_j p_j,
 // every shift j < num_now in one launch (blockIdx.y = j).
 template <typename T>
@@ -270,21 +285,80 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   using C = typename Spin<T>::C;
-  const C bj = (C)st->beta_i[j], aj = (C)st->alpha[j];
-  const double zj = st->zeta_ip1[j];
   cplx<C> xv[3], pv[3], rv[3];
   Spin<T>::load(x[j], i, xv);
   Spin<T>::load(p[j], i, pv);
   Spin<T>::load(r, i, rv);
-  for (int c = 0; c < 3; ++c) {
-    xv[c].re = xv[c].re + bj * pv[c].re;
-    xv[c].im = xv[c].im + bj * pv[c].im;
-    const C tr = (C)((double)rv[c].re * zj), ti = (C)((double)rv[c].im * zj);
-    pv[c].re = tr + aj * pv[c].re;
-    pv[c].im = ti + aj * pv[c].im;
+  ms_one<C>(xv, pv, rv, (C)st->beta_i[j], (C)st->alpha[j], st->zeta_ip1[j]);
+  Spin<T>::store(x[j], i, xv);
+  Spin<T>::store(p[j], i, pv);
+}
+
+// The same with the shifted systems updated up to st->fuse steps at a time.
+// Only the base system's p (j_low) feeds the operator, so on a deferred step
+// (st->defer, decided by this step's finalize) only j_low is updated and r is
+// kept in rprev slot st->pend; the next step that is not deferred applies, for
+// every other shift, the pending steps (their coefficients kept by
+// k_ms_commit) and then its own, reading and writing x_j and p_j once.  The
+// arithmetic and its order are those of k_ms_update_pm step by step.
+// flush = 1: the pending steps only (after the iteration stopped).
+template <typename T>
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_ms_update_pm_def(T *const *x, T *const *p, const T *r, T *rprev, int vh, const SolverState *st,
+                       const int *gate, int flush) {
+  if (!flush && *gate == 0) return;
+  const int j = blockIdx.y, jl = st->j_low, np = st->pend, nn = st->num_now;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  const size_t nv = 6 * (size_t)vh;
+  using C = typename Spin<T>::C;
+  cplx<C> xv[3], pv[3], rv[3];
+  if (j == jl) {  // the base system: every step, never pending
+    if (flush) return;
+    Spin<T>::load(r, i, rv);
+    if (st->defer) Spin<T>::store(rprev + (size_t)np * nv, i, rv);
+    Spin<T>::load(x[j], i, xv);
+    Spin<T>::load(p[j], i, pv);
+    ms_one<C>(xv, pv, rv, (C)st->beta_i[j], (C)st->alpha[j], st->zeta_ip1[j]);
+    Spin<T>::store(x[j], i, xv);
+    Spin<T>::store(p[j], i, pv);
+    return;
+  }
+  if (!flush && st->defer) return;
+  // num_now only shrinks, so the first pending step covers the later ones
+  if (j >= (np ? st->pend_nn[0] : nn)) return;
+  Spin<T>::load(x[j], i, xv);
+  Spin<T>::load(p[j], i, pv);
+  for (int q = 0; q < np; ++q) {
+    if (j >= st->pend_nn[q]) break;
+    Spin<T>::load(rprev + (size_t)q * nv, i, rv);
+    ms_one<C>(xv, pv, rv, (C)st->pbeta[q][j], (C)st->palpha[q][j], st->pzeta[q][j]);
+  }
+  if (!flush && j < nn) {
+    Spin<T>::load(r, i, rv);
+    ms_one<C>(xv, pv, rv, (C)st->beta_i[j], (C)st->alpha[j], st->zeta_ip1[j]);
   }
   Spin<T>::store(x[j], i, xv);
   Spin<T>::store(p[j], i, pv);
+}
+
+// End of a step of the deferring multi-shift update: keep this step's
+// coefficients when its shifted updates wait, else nothing is pending.
+// flush = 1: after k_ms_update_pm_def(flush) -- nothing pending.
+__global__ void k_ms_commit(SolverState *s, const int *gate, int flush) {
+  if (!flush && *gate == 0) return;
+  if (!flush && s->defer) {
+    const int q = s->pend++;
+    s->pend_nn[q] = s->num_now;
+    for (int j = 0; j < s->num_now; ++j) {
+      s->pbeta[q][j] = s->beta_i[j];
+      s->palpha[q][j] = s->alpha[j];
+      s->pzeta[q][j] = s->zeta_ip1[j];
+    }
+  } else {
+    s->pend = 0;
+  }
+  s->defer = 0;
 }
 
 // multi-shift update_dest: x_j += beta_j p_j, j < num_now
@@ -408,6 +482,16 @@ void blas_ms_update_pm(T *const *x, T *const *p, const T *r, int nshift, const S
 }
 
 template <typename T>
+void blas_ms_update_pm_def(T *const *x, T *const *p, const T *r, T *rprev, int nshift, SolverState *st,
+                           const int *gate, bool flush) {
+  const int vh = ctx().g.vh;
+  hipLaunchKernelGGL(k_ms_update_pm_def<T>, dim3(nb(vh), nshift), dim3(BLAS_BLOCK), 0, ctx().stream,
+                     x, p, r, rprev, vh, st, gate, flush ? 1 : 0);
+  hipLaunchKernelGGL(k_ms_commit, dim3(1), dim3(1), 0, ctx().stream, st, gate, flush ? 1 : 0);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
 void blas_ms_update_dest(T *const *x, T *const *p, int nshift, const SolverState *st) {
   const int vh = ctx().g.vh;
   hipLaunchKernelGGL(k_ms_update_dest<T>, dim3(nb(vh), nshift), dim3(BLAS_BLOCK), 0, ctx().stream,
@@ -429,7 +513,9 @@ void set_int(int *p, int v) {
   template void blas_ms_update_r<T>(T *, const T *, SolverState *, const int *);               \
   template void blas_ms_update_pm<T>(T *const *, T *const *, const T *, int,                   \
                                      const SolverState *, const int *);                        \
-  template void blas_ms_update_dest<T>(T *const *, T *const *, int, const SolverState *);
+  template void blas_ms_update_dest<T>(T *const *, T *const *, int, const SolverState *);      \
+  template void blas_ms_update_pm_def<T>(T *const *, T *const *, const T *, T *, int, SolverState *, \
+                                         const int *, bool);
 INST(double)
 INST(float)
 #undef INST

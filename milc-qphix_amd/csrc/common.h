@@ -92,6 +92,9 @@ void *dev_alloc(size_t bytes);
 // recorded (so no abort under abort-on-error).
 void *dev_try_alloc(size_t bytes);
 int fewsums_mode();
+// Most steps per update of the multi-shift solvers' shifted systems
+// (QPHIX_MI355X_set_msx_fuse; 1 = every step).
+int msx_fuse();
 void set_fewsums_mode(int mode);
 // Takes the device address of the no-allocation face scratch for `device`
 // (which must be current); once per device, at init.  False: error recorded.

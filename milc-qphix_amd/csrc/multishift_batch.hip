@@ -370,6 +370,7 @@ static int g_mrhs_lds = -1;
 // Steps per update of the shifted systems in the batched multi-shift solver
 // (k_msx_update_pm; identical results for every value): QPHIX_MI355X_set_msx_fuse.
 static int g_msx_fuse = MSX_MAXDEFER;
+int msx_fuse() { return g_msx_fuse; }
 static bool mrhs_lds_on() {
   if (g_mrhs_lds < 0) {
     const char *e = std::getenv("QPHIX_MRHS_LDS");

@@ -166,6 +166,8 @@ __device__ inline void apply_scalar_op(SolverState *s, int op, const double *v) 
     // The host never queues more than max_cg steps, so no cap is needed here.
     if (s->single_coef) ms_alpha_recurrence<float>(s);
     else ms_alpha_recurrence<double>(s);
+    // shifted updates of this step wait for a later one (k_ms_update_pm_def)
+    s->defer = (s->fuse > 1 && s->pend + 1 < s->fuse) ? 1 : 0;
     break;
   // Mixed precision: CG in the sloppy precision with reliable updates of the
   // FP64 solution and residual (Sleijpen & van der Vorst, Computing 56 (1996)

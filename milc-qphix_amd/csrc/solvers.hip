@@ -253,7 +253,9 @@ static int invert_multi(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *in
   const double t0 = wall_now();
   const int vh = ctx().g.vh;
   const size_t nv = 6 * (size_t)vh;
-  char *ws = (char *)ws_get((3 + (size_t)nshift) * nv * sizeof(T) + sizeof(SolverState) +
+  // shifted systems updated up to `fuse` steps at a time (fuse - 1 saved residuals)
+  const int fuse = msx_fuse();
+  char *ws = (char *)ws_get((3 + (size_t)nshift + (size_t)(fuse - 1)) * nv * sizeof(T) + sizeof(SolverState) +
                             2 * MAXNDMASS * sizeof(T *) + 512);
   if (!ws) {  // out of device memory, error recorded
     info->status = QPHIX_MEM_ERROR;
@@ -261,13 +263,15 @@ static int invert_multi(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *in
   }
   T *ttt = (T *)ws, *tmp = ttt + nv, *r = tmp + nv;
   T *pm0 = r + nv;
-  SolverState *dst = (SolverState *)(((uintptr_t)(pm0 + nshift * nv) + 255) & ~(uintptr_t)255);
+  T *rprev = pm0 + nshift * nv;
+  SolverState *dst = (SolverState *)(((uintptr_t)(rprev + (size_t)(fuse - 1) * nv) + 255) & ~(uintptr_t)255);
   T **dptr = (T **)(dst + 1);  // [0..nshift) x_j, [nshift..2nshift) p_j
   int *gate = &dst->active;
 
   SolverState hs{};
   hs.single_coef = sizeof(T) == 4;
   hs.nshift = nshift;
+  hs.fuse = fuse;
   double offset_low = 1.0e+20;
   int j_low = -1;
   for (int j = 0; j < nshift; ++j) {
@@ -315,6 +319,8 @@ static int invert_multi(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *in
     hs.num_now = nshift;
     hs.converged = 0;
     hs.active = 1;
+    hs.pend = 0;
+    hs.defer = 0;
     for (int j = 0; j < nshift; ++j) {
       hs.zeta_im1[j] = hs.zeta_i[j] = 1.0;
       hs.beta_im1[j] = -1.0;
@@ -337,7 +343,8 @@ static int invert_multi(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *in
         a2.scalar_op = SOP_MS_PKP;
         launch_dslash<T>(*L, parity, ttt, tmp, EPI_PKP, a2);
         blas_ms_update_r<T>(r, ttt, dst, gate);
-        blas_ms_update_pm<T>(dptr, dptr + nshift, r, nshift, dst, gate);
+        if (fuse > 1) blas_ms_update_pm_def<T>(dptr, dptr + nshift, r, rprev, nshift, dst, gate, false);
+        else blas_ms_update_pm<T>(dptr, dptr + nshift, r, nshift, dst, gate);
       }
     };
     BatchGraph steps;  // full batches; the last one, cut at max_cg, is enqueued directly
@@ -351,6 +358,8 @@ static int invert_multi(QPHIX_info_t *info, LinksT<T> *L, QPHIX_invert_arg_t *in
     }
     iteration = hs.iteration;
     total_iters += iteration;
+    // the shifted systems' pending steps, before the closing update
+    if (fuse > 1) blas_ms_update_pm_def<T>(dptr, dptr + nshift, r, rprev, nshift, dst, gate, true);
     if (hs.converged) blas_ms_update_dest<T>(dptr, dptr + nshift, nshift, dst);
     HIP_CHECK(hipStreamSynchronize(ctx().stream));
   }
