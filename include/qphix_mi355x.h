@@ -92,9 +92,10 @@ void QPHIX_MI355X_set_sloppy_link_bits(int bits);
 /* Long links of the FP32 sloppy operator (link bits 32): 8 (default) = each
  * long link as 8 reals (U01, U02, U10 and the half-angle tangents of the
  * phases of U00 and U20; the rest follows from unitarity and det = +-1), used
- * when the FP64 links store 12-real long links and no long link has
- * |U01|^2 + |U02|^2 < 1e-10, else the 12-real form; 12 = always the FP64 copy's
- * form.  ~1.5e-7 relative error per element typical, up to ~1e-4 at rare
+ * when the FP64 links store 12-real long links and every long link has rows 0
+ * and 1 orthonormal to 1e-6 and |U01|^2 + |U02|^2 >= 1e-10, else the 12-real
+ * form; 12 = always the FP64 copy's
+ * form.  ~1.5e-7 relative error per element typical, up to ~5e-5 at rare
  * links where |U00| or |U20| is tiny; the FP64 reliable updates keep the
  * result at the requested true residual.  Read at each solve. */
 void QPHIX_MI355X_set_sloppy_long_recon(int recon);

@@ -188,8 +188,10 @@ __device__ double half_tan(double x, double y) {
 
 // FP64 12-real long links -> FP32 8-real form: per link U01, U02, U10 and the
 // half-angle tangents of U00 and U20 = s conj(U01 U12 - U02 U11) (s the stored
-// row-2 sign).  Links with n = |U01|^2 + |U02|^2 < 1e-10 have no usable 8-real
-// form; they are counted in *bad and the caller keeps the 12-real form.
+// row-2 sign).  The 8-real rebuild takes |U00| and |U20| from unitarity, so a
+// link has no usable 8-real form when its rows 0 and 1 are not orthonormal to
+// 1e-6 (the 12-real check only ties row 2 to them) or n = |U01|^2 + |U02|^2 <
+// 1e-10; such links are counted in *bad and the caller keeps the 12-real form.
 __global__ void k_links_to_float8(float *dst, const double *src, const uint8_t *sign, int vh,
                                   unsigned *bad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -208,7 +210,14 @@ __global__ void k_links_to_float8(float *dst, const double *src, const uint8_t *
       const double xr = u[2] * u[10] - u[3] * u[11] - (u[4] * u[8] - u[5] * u[9]);
       const double xi = u[2] * u[11] + u[3] * u[10] - (u[4] * u[9] + u[5] * u[8]);
       const double n = u[2] * u[2] + u[3] * u[3] + u[4] * u[4] + u[5] * u[5];
-      if (!(n >= 1e-10)) ++nbad;
+      const double n0 = u[0] * u[0] + u[1] * u[1] + n;
+      const double n1 = u[6] * u[6] + u[7] * u[7] + u[8] * u[8] + u[9] * u[9] + u[10] * u[10] + u[11] * u[11];
+      double dr = 0, di = 0;  // row 0 . conj(row 1)
+      for (int c = 0; c < 3; ++c) {
+        dr += u[2 * c] * u[6 + 2 * c] + u[2 * c + 1] * u[6 + 2 * c + 1];
+        di += u[2 * c + 1] * u[6 + 2 * c] - u[2 * c] * u[6 + 2 * c + 1];
+      }
+      if (!(n >= 1e-10) || !(fabs(n0 - 1) <= 1e-6) || !(fabs(n1 - 1) <= 1e-6) || !(hypot(dr, di) <= 1e-6)) ++nbad;
       const float v[8] = {(float)u[2], (float)u[3], (float)u[4], (float)u[5], (float)u[6], (float)u[7],
                           (float)half_tan(u[0], u[1]), (float)half_tan(s * xr, -s * xi)};
       for (int k = 0; k < 4; ++k) {

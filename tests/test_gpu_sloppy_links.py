@@ -111,6 +111,48 @@ def test_sloppy_8real_falls_back_without_the_form():
         o.free()
 
 
+def test_sloppy_8real_falls_back_on_non_unitary_rows():
+    """Long links (D W) + 1/det W, with W a random unitary 2x2 block and D =
+    diag(1.01, 1/1.01), times random signs: row 2 equals s conj(row 0 x row 1)
+    for each link and its adjoint, so they store 12 reals, but rows 0 and 1
+    are not unit vectors, which the 8-real rebuild assumes (|U00| =
+    sqrt(1 - |U01|^2 - |U02|^2) would be wrong, or NaN).  The sloppy operator
+    keeps the 12-real form: both settings give the same output, within the
+    FP32 bar of the oracle."""
+    dims = (8, 8, 8, 8)
+    s = lattice_session(dims)
+    lat = F.Lattice(dims)
+    fat = random_links(lat, "haar", 1019)[0]
+    rng = np.random.default_rng(1021)
+    z = rng.standard_normal((lat.V, 4, 2, 2)) + 1j * rng.standard_normal((lat.V, 4, 2, 2))
+    w, _ = np.linalg.qr(z)
+    lng = np.zeros((lat.V, 4, 3, 3), np.complex128)
+    lng[..., :2, :2] = np.array([1.01, 1 / 1.01])[:, None] * w
+    lng[..., 2, 2] = 1 / np.linalg.det(w)
+    lng *= rng.choice([-1.0, 1.0], size=(lat.V, 4))[..., None, None]
+    raw = list(F.raw_link_set(lat, fat, lng))
+    orc = O.Oracle(dims, *raw)
+    L = A.Links(s, *raw)
+    assert s.fn("D", "asqtad_links_recon")(L.h) == 12
+    src = F.to_raw(F.gaussian_source(lat, 10), np.float32)
+    vin = A.Vector(s, src, A.EVENODD, "F")
+    vout = A.Vector(s, None, A.EVENODD, "F")
+    ref = orc.dslash(half(src.astype(np.float64), lat, 0), 1)
+    out = {}
+    try:
+        for recon in (8, 12):
+            print(f"[case] sloppy dslash non-unitary long links, long recon={recon}", flush=True)
+            s.lib.QPHIX_MI355X_set_sloppy_long_recon(recon)
+            assert A.dslash_sloppy(L, vout, vin, A.ODD, 32) > 0
+            out[recon] = half(vout.extract(), lat, 1)
+    finally:
+        s.lib.QPHIX_MI355X_set_sloppy_long_recon(8)
+    assert np.array_equal(out[8], out[12])
+    assert rel_l2(out[8], ref) < 1e-6
+    for o in (L, vin, vout):
+        o.free()
+
+
 def true_rsq(orc, lat, x, b, mass, cb):
     xh = half(x, lat, cb)
     bh = half(b, lat, cb)
