@@ -313,7 +313,9 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
 // residuals, 0.83 GB per step), 171 us with one block per slot in this order
 // (0.34 GB; a deferred step still dispatched every shift's blocks, 107 us for
 // 0.3 GB), 124 us with the resident grid (48 us per deferred step, 656 us per
-// full step; the solve 1.04 -> 0.96 s).
+// full step; the solve 1.04 -> 0.96 s).  One, three or four shifts at a time
+// instead of two: 664 / 646 / 638 us against 657 us per full step (168-239
+// VGPRs, no spills), under 0.3% of the solve.
 template <typename T>
 __device__ __forceinline__ void ms_def_one(T *const *x, T *const *p, const T *r, T *rprev, size_t nv,
                                            const SolverState *st, int j, int i, int flush) {
@@ -353,6 +355,7 @@ __global__ void __launch_bounds__(BLAS_BLOCK)
     k_ms_update_pm_def(T *const *x, T *const *p, const T *r, T *rprev, int vh, int nshift,
                        const SolverState *st, const int *gate, int flush) {
   if (!flush && *gate == 0) return;
+  if (flush && st->pend == 0) return;  // nothing pending
   const size_t nv = 6 * (size_t)vh;
   const int nsb = (vh + BLAS_BLOCK - 1) / BLAS_BLOCK;
   const int xc = blockIdx.x % 8, l = blockIdx.x / 8, per = gridDim.x / 8;
@@ -558,7 +561,7 @@ template <typename T>
 void blas_ms_update_pm_def(T *const *x, T *const *p, const T *r, T *rprev, int nshift, SolverState *st,
                            const int *gate, bool flush) {
   const int vh = ctx().g.vh;
-  // resident-sized grid, a multiple of 8 (one slab of slots per XCD)
+  // resident-sized grid, a multiple of 8 (one run of site blocks per XCD)
   const int grid = 8 * std::max(ctx().n_cu, 1);
   hipLaunchKernelGGL(k_ms_update_pm_def<T>, dim3(grid), dim3(BLAS_BLOCK), 0, ctx().stream, x, p, r, rprev, vh,
                      nshift, st, gate, flush ? 1 : 0);
