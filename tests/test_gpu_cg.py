@@ -129,13 +129,17 @@ def test_multishift_matches_oracle(prec):
         o.free()
 
 
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (4, 8, 4, 12), (4, 4, 4, 12)])
 @pytest.mark.parametrize("prec", ["D", "F"])
-def test_multishift_deferred_updates_match_single_steps(prec):
+def test_multishift_deferred_updates_match_single_steps(prec, dims):
     """QPHIX_*3_asqtad_invert_multi with its shifted systems updated up to 8 or
     3 steps at a time (the base system every step) gives the same solutions,
     iteration counts and residuals, bit for bit, as with every step; EVENODD,
-    so both parity passes start from a clean pending state."""
-    cfg = dict(CFG1, seed=1006)
+    so both parity passes start from a clean pending state.  The grouped
+    update splits a parity's blocks of 256 sites into 8 per-XCD ranges; the
+    lattices have 8, 3 and 1 such blocks (the last one holding 96 sites), so
+    some ranges are empty."""
+    cfg = dict(CFG1, dims=dims, seed=1006)
     s, lat, raw, L, orc = setup(cfg, prec)
     masses = [0.2, 0.025, 0.05, 0.1, 0.4]
     b = F.to_raw(F.gaussian_source(lat, 81), NP[prec])
@@ -145,7 +149,7 @@ def test_multishift_deferred_updates_match_single_steps(prec):
     try:
         for fuse in (8, 3, 1):
             s.lib.QPHIX_MI355X_set_msx_fuse(fuse)
-            print(f"[case] multishift prec={prec} shifted updates every {fuse} steps", flush=True)
+            print(f"[case] multishift prec={prec} dims={dims} shifted updates every {fuse} steps", flush=True)
             outs = [A.Vector(s, None, A.EVENODD, prec) for _ in masses]
             ret, info, res = A.invert_multi(L, outs, vb, masses, A.EVENODD, resid)
             assert info.status == 0
