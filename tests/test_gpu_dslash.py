@@ -74,6 +74,80 @@ def test_links_roundtrip_and_recon(prec, kind, recon):
     s.lib.QPHIX_MI355X_set_long_recon(0)
 
 
+@pytest.mark.parametrize("prec", ["D", "F"])
+@pytest.mark.parametrize("recon", [0, 18])
+@pytest.mark.parametrize("parity", [A.EVEN, A.ODD], ids=["even", "odd"])
+def test_single_parity_links(qlib, prec, recon, parity):
+    """asqtad_create_L_from_raw / extract_L_to_raw with QPHIX_EVEN or QPHIX_ODD
+    (qphix_d3.cpp:970-1206): the raw arrays are parity-blocked (even sites,
+    then odd), and only the given parity's block is read and written.  The
+    one-parity set stores the same links as the EVENODD set (same Naik form,
+    same decoded links) and gives a bit-identical Dslash onto its parity; the
+    Dslash onto the other parity and the solvers, which need both parities,
+    leave a recorded error and launch nothing."""
+    s = lattice_session((8, 8, 8, 8))
+    lat, links, raw = links_for(s.dims, "haar")
+    rawp = [r.astype(NP[prec]) for r in raw]
+    cb = 0 if parity == A.EVEN else 1
+    n = 72 * lat.V // 2
+    mine, other = slice(cb * n, (cb + 1) * n), slice((1 - cb) * n, (2 - cb) * n)
+    # the other parity's blocks hold garbage: the import must not read them
+    junk = [r.copy() for r in rawp]
+    for r in junk:
+        r[other] = np.nan
+    qlib.QPHIX_MI355X_set_long_recon(recon)
+    try:
+        L1 = A.Links(s, *junk, parity=parity, prec=prec)
+        Lb = A.Links(s, *rawp, prec=prec)
+    finally:
+        qlib.QPHIX_MI355X_set_long_recon(0)
+    print(f"[case] single-parity links prec={prec} recon={recon} parity={parity}", flush=True)
+    assert L1.recon == Lb.recon == (12 if recon == 0 else 18)
+    fat, lng = L1.extract(parity)
+    fat_b, lng_b = Lb.extract(A.EVENODD)
+    np.testing.assert_array_equal(fat[mine], rawp[0][mine])
+    np.testing.assert_array_equal(lng[mine], lng_b[mine])
+    assert not fat[other].any() and not lng[other].any()
+    if recon == 18:
+        np.testing.assert_array_equal(lng[mine], rawp[1][mine])
+    else:
+        assert np.max(np.abs(lng[mine] - rawp[1][mine])) < (1e-14 if prec == "D" else 2e-6)
+    src = F.to_raw(F.gaussian_source(lat, 17), NP[prec])
+    vin = A.Vector(s, src, A.EVENODD, prec)
+    v1 = A.Vector(s, None, A.EVENODD, prec)
+    vb = A.Vector(s, None, A.EVENODD, prec)
+    A.dslash(L1, v1, vin, parity)
+    A.dslash(Lb, vb, vin, parity)
+    np.testing.assert_array_equal(half(v1.extract(), lat, cb), half(vb.extract(), lat, cb))
+    qlib.QPHIX_MI355X_set_abort_on_error(0)
+    qlib.QPHIX_MI355X_clear_error()
+    try:
+        before = v1.extract()
+        A.dslash(L1, v1, vin, A.ODD if parity == A.EVEN else A.EVEN)
+        nerr, msg = qlib.last_error()
+        assert nerr > 0 and "parity missing" in msg, msg
+        np.testing.assert_array_equal(v1.extract(), before)
+        qlib.QPHIX_MI355X_clear_error()
+        b = F.to_raw(F.gaussian_source(lat, 18, parity=cb), NP[prec])
+        vbs = A.Vector(s, b, parity, prec)
+        vx = A.Vector(s, None, parity, prec)
+        ret, info, _ = A.invert(L1, vx, vbs, 0.05, parity, 1e-12)
+        nerr, msg = qlib.last_error()
+        assert ret == 0 and info.status != 0 and nerr > 0 and "both link parities" in msg, msg
+        qlib.QPHIX_MI355X_clear_error()
+        outs = [A.Vector(s, None, parity, prec) for _ in range(2)]
+        ret, info, _ = A.invert_multi(L1, outs, vbs, [0.05, 0.1], parity, 1e-12)
+        nerr, msg = qlib.last_error()
+        assert ret == 0 and nerr > 0 and "both link parities" in msg, msg
+        for o in outs + [vbs, vx]:
+            o.free()
+    finally:
+        qlib.QPHIX_MI355X_clear_error()
+        qlib.QPHIX_MI355X_set_abort_on_error(1)
+    for o in (L1, Lb, vin, v1, vb):
+        o.free()
+
+
 def test_non_unitary_long_links_stay_18():
     s = lattice_session((8, 8, 8, 8))
     lat = F.Lattice(s.dims)
