@@ -3,7 +3,8 @@
 Bar: the same true residual is reached, iteration counts within 5% (the
 library reduces in FP64 where the reference's F build reduces in float, so F
 counts may differ slightly), FP64 solutions within 1e-9 relative L2 of the
-oracle's, FP32 solutions within 1e-4 of the FP64 answer.
+oracle's; FP32 solutions within 10x the error that the oracle's FP32
+restatement itself makes against the FP64 solution.
 """
 import numpy as np
 import pytest
@@ -67,10 +68,15 @@ def test_cg_config1_matches_oracle(prec, parity):
         assert io["final_rsq"] < cfg["resid"]
         assert abs(np.log10(res.final_rsq / io["final_rsq"])) < 0.3
     else:
-        # pure FP32 stagnates near eps_float * cond(M) (~1e-4 relative here),
-        # as the reference's F build does
+        # pure FP32 stagnates above the target, as the reference's F build
+        # does; against the FP64 solution the GPU's FP32 solution must be
+        # within 10x the error of the oracle's FP32 restatement (7e-7 here)
         assert tr < 1e-6
-        assert rel_l2(half(x, lat, cb), half(xo, lat, cb)) < 2e-3
+        x64, _ = orc64.invert(b.astype(np.float64), cfg["mass"], parity, 1e-24)
+        e_ref = rel_l2(half(xo, lat, cb).astype(np.float64), half(x64, lat, cb))
+        e_gpu = rel_l2(half(x, lat, cb).astype(np.float64), half(x64, lat, cb))
+        print(f"[case] FP32 error vs FP64: library {e_gpu:.2e}, oracle {e_ref:.2e}", flush=True)
+        assert e_gpu < 10 * max(e_ref, 1e-6)
     assert info.final_flop == pytest.approx(1187.0 * res.final_iter * lat.V)
     for o in (vb, vx, L):
         o.free()
@@ -116,13 +122,19 @@ def test_multishift_matches_oracle(prec):
     assert abs(ret - io["ret"]) <= max(1, 0.05 * io["ret"])
     assert all(r.final_iter == ret for r in res)
     orc64 = O.Oracle(lat.dims, *raw)
+    if prec == "F":  # the FP64 solutions, for the FP32 error bar
+        x64, _ = orc64.invert_multi(b.astype(np.float64), np.array(masses), O.EVEN, 1e-24)
     for j, m in enumerate(masses):
         x = outs[j].extract()
         if prec == "D":
             assert rel_l2(half(x, lat, 0), half(xo[j], lat, 0)) < 1e-9
             assert true_rsq(orc64, lat, x, b, m, 0) < 100 * resid
         else:
-            assert rel_l2(half(x, lat, 0), half(xo[j], lat, 0)) < 2e-3
+            # within 10x the error of the oracle's FP32 restatement (1e-6 to 1.3e-5 here)
+            e_ref = rel_l2(half(xo[j], lat, 0).astype(np.float64), half(x64[j], lat, 0))
+            e_gpu = rel_l2(half(x, lat, 0).astype(np.float64), half(x64[j], lat, 0))
+            print(f"[case] shift {m}: FP32 error vs FP64: library {e_gpu:.2e}, oracle {e_ref:.2e}", flush=True)
+            assert e_gpu < 10 * max(e_ref, 1e-6)
             assert true_rsq(orc64, lat, x, b, m, 0) < 1e-5
     assert info.final_flop == pytest.approx((1205.0 + 15.0 * len(masses)) * ret * lat.V)
     for o in outs + [vb, L]:
