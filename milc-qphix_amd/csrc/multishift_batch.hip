@@ -539,6 +539,34 @@ static double wall_now() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+// FP64 true residual of one source from its D D x (dd, overwritten with r):
+// r = b + (dd + s x), the arithmetic and order of the Dslash's restart
+// epilogue; |r|^2 and the relative-residue term reduced through rc.
+__global__ void __launch_bounds__(BLAS_BLOCK)
+    k_msx_true_res(double *dd, const double *x, const double *b, double s, int vh, ReduceCtl rc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < vh) {
+    cplx<double> acc[3], xv[3], bv[3];
+    Spin<double>::load(dd, i, acc);
+    Spin<double>::load(x, i, xv);
+    Spin<double>::load(b, i, bv);
+    double num = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double tr = acc[c].re + s * xv[c].re;
+      const double ti = acc[c].im + s * xv[c].im;
+      acc[c].re = bv[c].re + tr;
+      acc[c].im = bv[c].im + ti;
+      num += acc[c].re * acc[c].re + acc[c].im * acc[c].im;
+    }
+    Spin<double>::store(dd, i, acc);
+    v[0] = num;
+    v[1] = rel_term(site_norm2_rounded(acc), site_norm2_rounded(xv));
+  }
+  grid_reduce<2>(v, rc);
+}
+
 #include "multishift_driver.h"
 
 // ----------------------------------------------------------------- multi-RHS Dslash API

@@ -82,6 +82,20 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     w64 = (double *)dev_alloc((2 * (size_t)nshift + 3 + NRP) * nvr * sizeof(double));
     if (!w64) return false;
   }
+  // Optional workspace for the FP64 true residuals of several paused sources
+  // at once (their x_low packed, D x and D D x by the multi-RHS Dslash, then
+  // unpacked): packed in, packed out, gate states, result states.  Without it
+  // each source takes its own pair of Dslashes.
+  struct TrBuf {
+    void *p = nullptr;
+    ~TrBuf() { dev_free(p); }
+  } trbuf;
+  const size_t trb = nrhs <= 2 ? 2 : (nrhs <= 4 ? 4 : 8);  // packed block of the checked sources
+  trbuf.p = nrhs > 1 ? dev_try_alloc(2 * trb * nv * sizeof(double) + 16 * sizeof(SolverState) + 512) : nullptr;
+  double *tr_in = (double *)trbuf.p, *tr_out = tr_in ? tr_in + trb * nv : nullptr;
+  SolverState *tr_gate =
+      tr_in ? (SolverState *)(((uintptr_t)(tr_out + trb * nv) + 255) & ~(uintptr_t)255) : nullptr;
+  SolverState *tr_st = tr_gate ? tr_gate + 8 : nullptr;
   double *p_d = w64, *xs_d = hi ? p_d + nshift * nvr : nullptr;
   double *rs_d = hi ? xs_d + nshift * nvr : nullptr, *Ap_d = hi ? rs_d + nvr : nullptr;
   double *tmp_d = hi ? Ap_d + nvr : nullptr, *rprev_d = hi ? tmp_d + nvr : nullptr;
@@ -204,8 +218,9 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
     std::memcpy(&t, ctx().pinned, sizeof(t));
     return t;
   };
-  // Source r after a true-residual check: finished, or resumed in FP32.
-  auto settle = [&](int r, const SolverState &t, double new_delta) {
+  // Source r after a true-residual check (state t, and on the device its
+  // residual rr and state td): finished, or resumed in FP32.
+  auto settle = [&](int r, const SolverState &t, double new_delta, const double *rr, const SolverState *td) {
     bs.ms_iters[r] = hs[r].iteration;
     bs.final_rsq[r] = t.final_rsq;
     if (t.final_rsq < resid_src[r] || hs[r].iteration >= max_cg || !std::isfinite(t.rsq)) {
@@ -215,8 +230,44 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
       return;
     }
     ++bs.updates[r];
-    hipLaunchKernelGGL(k_msx_set_r, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, r64, vh, NR, r);
-    hipLaunchKernelGGL(k_msx_resume, dim3(1), dim3(1), 0, ctx().stream, st + r, tst, new_delta);
+    hipLaunchKernelGGL(k_msx_set_r, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, rs, rr, vh, NR, r);
+    hipLaunchKernelGGL(k_msx_resume, dim3(1), dim3(1), 0, ctx().stream, st + r, td, new_delta);
+  };
+  // True-residual checks of the paused sources R (their x already flushed),
+  // then settle.  Two or more share the multi-RHS Dslash: x_low packed, D x,
+  // D D x, unpacked, and r = b + D D x - sigma_low x per source, one host read
+  // for all.
+  auto check = [&](const std::vector<int> &R, double new_delta) {
+    const int n = (int)R.size();
+    if (n < 2 || !tr_in) {
+      for (int r : R) settle(r, true_residual(r), new_delta, r64, tst);
+      return;
+    }
+    const double *src[8];
+    char *pin = (char *)ctx().pinned;
+    for (int k = 0; k < n; ++k) {
+      src[k] = x64[R[k] * nshift + j_low];
+      SolverState t{};
+      t.src_norm = hs[R[k]].src_norm;
+      std::memcpy(pin + k * sizeof(SolverState), &t, sizeof(t));
+    }
+    HIP_CHECK(hipMemcpyAsync(tr_st, pin, n * sizeof(SolverState), hipMemcpyHostToDevice, ctx().stream));
+    const int nr = mrhs_dslash_packed<double>(*L, other, src, n, tr_in, tr_out, tr_gate);  // D x
+    if (nr == 2) mrhs_dslash<2, double>(*L, parity, tr_in, tr_out, EPI_PLAIN, 0.0, nullptr, tr_gate, 1);
+    else if (nr == 4) mrhs_dslash<4, double>(*L, parity, tr_in, tr_out, EPI_PLAIN, 0.0, nullptr, tr_gate, 1);
+    else mrhs_dslash<8, double>(*L, parity, tr_in, tr_out, EPI_PLAIN, 0.0, nullptr, tr_gate, 1);
+    mrhs_unpack<double>(tr_in, tr_out, n, nr);  // D D x of source k at tr_out + k nv
+    for (int k = 0; k < n; ++k) {
+      const ReduceCtl rc = reduce_ctl(tr_st + k, SOP_MX_TRUE);
+      hipLaunchKernelGGL(k_msx_true_res, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, tr_out + k * nv, src[k],
+                         b64[R[k]], -offset_low, vh, rc);
+      reduce_finish(2, nb, rc, nullptr);
+    }
+    HIP_CHECK(hipMemcpyAsync(pin, tr_st, n * sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
+    HIP_CHECK(hipStreamSynchronize(ctx().stream));
+    std::vector<SolverState> ts(n);
+    for (int k = 0; k < n; ++k) std::memcpy(&ts[k], pin + k * sizeof(SolverState), sizeof(SolverState));
+    for (int k = 0; k < n; ++k) settle(R[k], ts[k], new_delta, tr_out + k * nv, tr_st + k);
   };
 
   while (true) {
@@ -242,12 +293,14 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
       for (int r = 0; r < nrhs; ++r)
         if (!done[r] && hs[r].active) all = false;
       if (!all) continue;
+      std::vector<int> R;
       for (int r = 0; r < nrhs; ++r) {
         if (done[r]) continue;
         hipLaunchKernelGGL(k_msx_flush<double>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, dx + r * nshift,
                            xs_d, vh, NR, r, nshift);
-        settle(r, true_residual(r), delta);
+        R.push_back(r);
       }
+      check(R, delta);
       const size_t np = (size_t)nshift * nvr;  // reals
       const size_t nslot = np / 6;             // spinor slots (site x source, every shift)
       hipLaunchKernelGGL(k_msx_demote, dim3((unsigned)((nslot + BLAS_BLOCK - 1) / BLAS_BLOCK)), dim3(BLAS_BLOCK), 0,
@@ -260,13 +313,15 @@ static bool run_chunk(LinksT<double> *L, int parity, int nrhs, int nshift, const
       steps.reset();
       continue;
     }
+    std::vector<int> R;
     for (int r = 0; r < nrhs; ++r) {
       if (done[r] || hs[r].active) continue;
       // reliable update of source r: flush x, FP64 true residual of the base system
       hipLaunchKernelGGL(k_msx_flush<float>, dim3(nb), dim3(BLAS_BLOCK), 0, ctx().stream, dx + r * nshift, xs,
                          vh, NR, r, nshift);
-      settle(r, true_residual(r), 0.0);
+      R.push_back(r);
     }
+    check(R, 0.0);
   }
   HIP_CHECK(hipStreamSynchronize(ctx().stream));
   dev_free(w64);
